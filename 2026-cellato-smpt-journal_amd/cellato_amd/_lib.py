@@ -1,0 +1,83 @@
+"""ctypes binding of the C ABI in include/cellato_hip.h (libcellato_hip.so, built in-tree).
+
+The library is the only compute path: if it is missing this module raises instead of
+falling back to anything else.
+"""
+import ctypes
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "libcellato_hip.so"
+
+OK = 0
+E_RULE, E_LAYOUT, E_GEOMETRY, E_TEMPORAL, E_ARG, E_HIP = -1, -2, -3, -4, -5, -6
+LAYOUT_LINEAR, LAYOUT_TILED = 0, 1
+
+_P64 = ctypes.POINTER(ctypes.c_uint64)
+_PP64 = ctypes.POINTER(_P64)
+
+
+class CellatoError(RuntimeError):
+    """A failed C-ABI call: `code` is the negative status, the message comes from the library."""
+
+    def __init__(self, code, message):
+        super().__init__(f"{message} (status {code})")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libcellato_hip.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "(make -C 2026-cellato-smpt-journal_amd/csrc)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.cellato_rule_count.restype = ctypes.c_int
+        L.cellato_rule_name.restype = ctypes.c_char_p
+        L.cellato_rule_name.argtypes = [ctypes.c_int]
+        L.cellato_rule_find.argtypes = [ctypes.c_char_p]
+        L.cellato_rule_planes.argtypes = [ctypes.c_int]
+        L.cellato_rule_states.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+        L.cellato_max_temporal_steps.restype = ctypes.c_int
+        L.cellato_hip_last_error.restype = ctypes.c_char_p
+        L.cellato_hip_simple.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, _PP64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_int)]
+        L.cellato_hip_temporal.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, _PP64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                           ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise CellatoError(rc, lib().cellato_hip_last_error().decode())
+    return rc
+
+
+def plane_array(ptrs):
+    """Host array of device plane pointers (uint64_t* const*)."""
+    arr = (_P64 * len(ptrs))(*[ctypes.cast(ctypes.c_void_p(int(p)), _P64) for p in ptrs])
+    return arr
+
+
+def simple(rule, layout, ptrs_a, ptrs_b, width, height, t0, steps, chunk_rows=0, stream=None):
+    """cellato_hip_simple; returns True when the result is in buffer b."""
+    res = ctypes.c_int(-1)
+    check(lib().cellato_hip_simple(rule, layout, plane_array(ptrs_a), plane_array(ptrs_b), width, height, t0,
+                                   steps, chunk_rows, ctypes.c_void_p(stream or 0), ctypes.byref(res)))
+    return bool(res.value)
+
+
+def temporal(rule, layout, ptrs_a, ptrs_b, width, height, t0, steps, temporal_steps, chunk_rows=0, stream=None):
+    """cellato_hip_temporal; returns True when the result is in buffer b."""
+    res = ctypes.c_int(-1)
+    check(lib().cellato_hip_temporal(rule, layout, plane_array(ptrs_a), plane_array(ptrs_b), width, height, t0,
+                                     steps, temporal_steps, chunk_rows, ctypes.c_void_p(stream or 0),
+                                     ctypes.byref(res)))
+    return bool(res.value)

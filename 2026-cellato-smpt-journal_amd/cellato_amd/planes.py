@@ -1,0 +1,68 @@
+"""Bit-plane grids on the host and on the GPU.
+
+`PlaneGrid` mirrors the reference's bit_planes::grid / tiled_bit_planes::grid
+(include/memory/bit_planes_grid.hpp:30-271, tiled_bit_planes_grid.hpp:30-289): P planes of
+64-bit words, built from a standard grid of state values, convertible back, with the
+reference's checksum. `DeviceGrid` holds the two ping-pong buffers of a grid in HBM (one
+torch allocation, planes as views), the way the reference traversers own two device grids.
+"""
+import numpy as np
+
+from . import grid as hg
+from .rules import rule as get_rule
+
+
+class PlaneGrid:
+    def __init__(self, rule, width, height, planes, layout=hg.LINEAR):
+        self.rule = get_rule(rule) if not hasattr(rule, "planes") else rule
+        self.width, self.height, self.layout = width, height, layout
+        self.planes = planes   # (P, rows, words) uint64
+
+    @classmethod
+    def from_standard(cls, rule, values, layout=hg.LINEAR):
+        r = get_rule(rule) if not hasattr(rule, "planes") else rule
+        v = np.asarray(values)
+        H, W = v.shape
+        return cls(r, W, H, hg.pack(r.index_of(v), r.planes, layout), layout)
+
+    def to_standard(self):
+        return self.rule.value_of(hg.unpack(self.planes, self.width, self.height, self.layout))
+
+    def x_size_physical(self):
+        return self.planes.shape[2]
+
+    def y_size_physical(self):
+        return self.planes.shape[1]
+
+    def get_checksum(self):
+        return hg.checksum_str(self.to_standard())
+
+
+class DeviceGrid:
+    """Two device buffers (a, b) of P planes each; `cur` says which one holds the state."""
+
+    def __init__(self, rule, width, height, layout=hg.LINEAR, device="cuda"):
+        import torch
+        self.rule = get_rule(rule) if not hasattr(rule, "planes") else rule
+        self.width, self.height, self.layout = width, height, layout
+        rows, words = hg.plane_shape(layout, width, height)
+        self.shape = (self.rule.planes, rows, words)
+        self.buf = torch.zeros((2,) + self.shape, dtype=torch.int64, device=device)
+        self.cur = 0
+
+    @property
+    def nbytes_state(self):
+        return int(np.prod(self.shape)) * 8
+
+    def ptrs(self, which):
+        return [self.buf[which, p].data_ptr() for p in range(self.shape[0])]
+
+    def upload(self, planes):
+        import torch
+        self.buf[self.cur].copy_(torch.from_numpy(np.ascontiguousarray(planes).view(np.int64)))
+
+    def download(self):
+        return self.buf[self.cur].cpu().numpy().view(np.uint64).copy()
+
+    def to_plane_grid(self):
+        return PlaneGrid(self.rule, self.width, self.height, self.download(), self.layout)

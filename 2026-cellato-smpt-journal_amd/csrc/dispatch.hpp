@@ -1,0 +1,88 @@
+// Build-time instantiation table: for every automaton, the stream kernel instantiated for
+// each (layout, lane-exchange, halo, T) the host launcher can pick. Plays the role of the
+// reference's explicit-instantiation glue (src/_shared/cuda_instantiation_template.cuh:19-193,
+// included from traversers/cuda/simple.cu:101-114 and temporal.cu) and of its run-time ->
+// compile-time dispatcher (include/utils/static_dispatcher.hpp:27-92): here the host code
+// indexes a table instead of searching option sets.
+#ifndef CELLATO_HIP_DISPATCH_HPP
+#define CELLATO_HIP_DISPATCH_HPP
+
+#include "stream.hpp"
+
+namespace cellato::hip {
+
+constexpr int MAX_T = 8;   // generations per launch of the temporal traverser
+
+enum simple_variant : int {
+    SIMPLE_SEGMENT = 0,    // rows of 64k columns: DPP, end lanes load the outside column
+    SIMPLE_SEGMENT_ANY = 1,// rows of other widths > 64: bpermute, same halo loads
+    SIMPLE_RING64 = 2,     // rows of exactly 64 columns
+    SIMPLE_GROUPS = 3,     // rows of fewer than 64 columns, several rows per wave
+    SIMPLE_VARIANTS = 4
+};
+
+enum temporal_variant : int {
+    TEMPORAL_SEGMENT = 0,  // rows wider than 64 columns: 62 owned + 2 halo lanes per wave
+    TEMPORAL_RING64 = 1,
+    TEMPORAL_GROUPS = 2,
+    TEMPORAL_VARIANTS = 3
+};
+
+struct rule_kernels {
+    const char* name;
+    int planes;
+    int nstates;                 // states of the dictionary
+    int state_values[32];        // enum value of each dictionary index (for checksums)
+    const void* simple[2][SIMPLE_VARIANTS];                   // [layout][variant]
+    const void* temporal[2][TEMPORAL_VARIANTS][MAX_T + 1];    // [layout][variant][T], T >= 1
+};
+
+template <class Eval, int L>
+constexpr void fill_layout(rule_kernels& k) {
+    k.simple[L][SIMPLE_SEGMENT] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_SEGMENT, HALO_LOAD>);
+    k.simple[L][SIMPLE_SEGMENT_ANY] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_PERMUTE, HALO_LOAD>);
+    k.simple[L][SIMPLE_RING64] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_RING64, HALO_NONE>);
+    k.simple[L][SIMPLE_GROUPS] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_PERMUTE, HALO_NONE>);
+    k.temporal[L][TEMPORAL_SEGMENT][0] = nullptr;
+    k.temporal[L][TEMPORAL_RING64][0] = nullptr;
+    k.temporal[L][TEMPORAL_GROUPS][0] = nullptr;
+    [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
+        ((k.temporal[L][TEMPORAL_SEGMENT][Ts + 1] =
+              reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_SEGMENT, HALO_LANES>)), ...);
+        ((k.temporal[L][TEMPORAL_RING64][Ts + 1] =
+              reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_RING64, HALO_NONE>)), ...);
+        ((k.temporal[L][TEMPORAL_GROUPS][Ts + 1] =
+              reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_PERMUTE, HALO_NONE>)), ...);
+    }(std::make_integer_sequence<int, MAX_T>{});
+}
+
+template <class Config>
+rule_kernels make_rule_kernels() {
+    using Eval = cellato::evaluators::bit_planes::evaluator<word, typename Config::state_dictionary, typename Config::algorithm>;
+    rule_kernels k{};
+    k.name = Config::name;
+    k.planes = Config::state_dictionary::needed_bits;
+    k.nstates = Config::state_dictionary::number_of_values;
+    for (int i = 0; i < k.nstates && i < 32; ++i)
+        k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
+    fill_layout<Eval, LINEAR>(k);
+    fill_layout<Eval, TILED>(k);
+    return k;
+}
+
+// one per automaton, defined in rule_<id>.hip
+const rule_kernels& kernels_of_rule(int id);
+const rule_kernels& kernels_rule_0();
+const rule_kernels& kernels_rule_1();
+const rule_kernels& kernels_rule_2();
+const rule_kernels& kernels_rule_3();
+const rule_kernels& kernels_rule_4();
+const rule_kernels& kernels_rule_5();
+const rule_kernels& kernels_rule_6();
+const rule_kernels& kernels_rule_7();
+const rule_kernels& kernels_rule_8();
+const rule_kernels& kernels_rule_9();
+
+} // namespace cellato::hip
+
+#endif

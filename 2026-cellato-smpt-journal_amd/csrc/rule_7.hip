@@ -1,0 +1,10 @@
+// Kernel instantiations for automaton id 7 of include/cellato/automata/all.hpp.
+#include "cellato/automata/all.hpp"
+#include "dispatch.hpp"
+
+namespace cellato::hip {
+const rule_kernels& kernels_rule_7() {
+    static const rule_kernels k = make_rule_kernels<cellato::automata::by_id<7>::config>();
+    return k;
+}
+} // namespace cellato::hip

@@ -1,0 +1,501 @@
+// Bit-plane evaluator: lowers a rule type (cellato::ast) into bitwise logic over 64-bit
+// words, 64 cells per word and one word per bit plane. Same role and name as the
+// reference's evaluators/bit_planes.hpp (`evaluator<word, dictionary, Expression>`), but
+// the evaluation does not read memory itself. It reads a *window*: the caller (a HIP
+// kernel or a CPU loop) holds the words around the word being evaluated and answers
+//
+//   template <int dx, int dy> vint<P> nb() const        value of neighbour (dx, dy), all planes
+//   template <int dx, int dy, class F> word nbf(F) const F(value) for a per-cell F, computed
+//                                                         before the shift (cheaper for P > 1)
+//   word yodd() const                                     cells whose global row is odd
+//   int t                                                 time step of the generation computed
+//
+// so one evaluator serves the linear (64x1 cells per word) and the tiled (8x8) layouts and
+// any register/LDS staging. Results are exact: a value is a bit-sliced unsigned integer
+// wide enough for every intermediate, so the result equals the reference's standard
+// evaluator (include/evaluators/standard.hpp) on the dictionary index of every cell.
+#ifndef CELLATO_EVALUATORS_BIT_PLANES_HPP
+#define CELLATO_EVALUATORS_BIT_PLANES_HPP
+
+#include <cstdint>
+#include <type_traits>
+
+#include "../core/ast.hpp"
+
+#if defined(__HIPCC__)
+#define CELLATO_HD __host__ __device__ __forceinline__
+#else
+#define CELLATO_HD inline
+#endif
+
+namespace cellato::evaluators::bit_planes {
+
+using word = std::uint64_t;
+using namespace cellato::ast;
+
+// ---- bit-sliced unsigned integers ---------------------------------------------------------
+template <int N>
+struct vint {
+    static_assert(N >= 1, "a value has at least one plane");
+    static constexpr int bits = N;
+    word p[N];
+};
+
+template <class T> struct is_vint : std::false_type {};
+template <int N> struct is_vint<vint<N>> : std::true_type {};
+
+template <class T> struct is_state_literal : std::false_type {};
+template <auto V> struct is_state_literal<cellato::ast::state_constant<V>> : std::true_type {};
+
+constexpr int bit_width(long long v) { return v <= 0 ? 0 : 1 + bit_width(v >> 1); }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <long long V>
+CELLATO_HD vint<cmax(1, bit_width(V))> make_const() {
+    constexpr int N = cmax(1, bit_width(V));
+    vint<N> r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.p[i] = ((V >> i) & 1) ? ~word(0) : word(0);
+    return r;
+}
+
+template <int N>
+CELLATO_HD word plane(const vint<N>& a, int i) { return i < N ? a.p[i] : word(0); }
+
+template <int K, int N>
+CELLATO_HD vint<K> resize(const vint<N>& a) {
+    vint<K> r;
+#pragma unroll
+    for (int i = 0; i < K; ++i) r.p[i] = plane(a, i);
+    return r;
+}
+
+// A mask (one word, bit = condition of that cell) used as a 1-plane value.
+CELLATO_HD vint<1> as_vint(word m) { vint<1> r; r.p[0] = m; return r; }
+template <int N> CELLATO_HD vint<N> as_vint(const vint<N>& v) { return v; }
+
+// Three-input boolean function `lut` (truth table indexed by a*4 + b*2 + c) per bit; one
+// v_bitop3_b32 per 32-bit half on gfx950.
+template <unsigned lut>
+CELLATO_HD word lop3(word a, word b, word c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned lo = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(a), static_cast<unsigned>(b), static_cast<unsigned>(c), lut);
+    const unsigned hi = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(a >> 32), static_cast<unsigned>(b >> 32), static_cast<unsigned>(c >> 32), lut);
+    return (word(hi) << 32) | lo;
+#else
+    word r = 0;
+    for (int i = 0; i < 8; ++i)
+        if ((lut >> i) & 1) r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+    return r;
+#endif
+}
+
+// full and half adders on words (sum = xor3, carry = majority)
+CELLATO_HD void fa(word a, word b, word c, word& s, word& cy) {
+    s = lop3<0x96>(a, b, c);
+    cy = lop3<0xE8>(a, b, c);
+}
+CELLATO_HD void ha(word a, word b, word& s, word& cy) { s = a ^ b; cy = a & b; }
+
+// exact sum: one plane wider than the wider operand
+template <int A, int B>
+CELLATO_HD vint<cmax(A, B) + 1> add(const vint<A>& a, const vint<B>& b) {
+    constexpr int R = cmax(A, B);
+    vint<R + 1> r;
+    word carry = 0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) fa(plane(a, i), plane(b, i), carry, r.p[i], carry);
+    r.p[R] = carry;
+    return r;
+}
+
+template <int A, int B>
+CELLATO_HD word eq(const vint<A>& a, const vint<B>& b) {
+    word r = ~word(0);
+#pragma unroll
+    for (int i = 0; i < cmax(A, B); ++i) r &= ~(plane(a, i) ^ plane(b, i));
+    return r;
+}
+
+template <long long K, int A>
+CELLATO_HD word eq_const(const vint<A>& a) {
+    if constexpr (bit_width(K) > A) {
+        return word(0);
+    } else {
+        word r = ~word(0);
+#pragma unroll
+        for (int i = 0; i < A; ++i) r &= ((K >> i) & 1) ? a.p[i] : ~a.p[i];
+        return r;
+    }
+}
+
+// a > b per cell, scanning from the most significant plane
+template <int A, int B>
+CELLATO_HD word gt(const vint<A>& a, const vint<B>& b) {
+    word res = 0, undecided = ~word(0);
+#pragma unroll
+    for (int i = cmax(A, B) - 1; i >= 0; --i) {
+        const word x = plane(a, i), y = plane(b, i);
+        res |= undecided & x & ~y;
+        undecided &= ~(x ^ y);
+    }
+    return res;
+}
+
+template <int A, int B, class Op>
+CELLATO_HD vint<cmax(A, B)> zip(const vint<A>& a, const vint<B>& b, Op op) {
+    vint<cmax(A, B)> r;
+#pragma unroll
+    for (int i = 0; i < cmax(A, B); ++i) r.p[i] = op(plane(a, i), plane(b, i));
+    return r;
+}
+
+// per-cell select: m ? a : b
+template <int A, int B>
+CELLATO_HD vint<cmax(A, B)> select(word m, const vint<A>& a, const vint<B>& b) {
+    vint<cmax(A, B)> r;
+#pragma unroll
+    for (int i = 0; i < cmax(A, B); ++i) r.p[i] = (plane(a, i) & m) | (plane(b, i) & ~m);
+    return r;
+}
+CELLATO_HD word select(word m, word a, word b) { return (a & m) | (b & ~m); }
+
+// population count of 8 masks as a 4-plane value (carry-save adder tree)
+CELLATO_HD vint<4> count8(const word (&m)[8]) {
+    word s1, c1, s2, c2, s3, c3, c4, s5, c5, c6;
+    vint<4> r;
+    fa(m[0], m[1], m[2], s1, c1);
+    fa(m[3], m[4], m[5], s2, c2);
+    ha(m[6], m[7], s3, c3);
+    fa(s1, s2, s3, r.p[0], c4);
+    fa(c1, c2, c3, s5, c5);
+    ha(s5, c4, r.p[1], c6);
+    ha(c5, c6, r.p[2], r.p[3]);
+    return r;
+}
+
+CELLATO_HD vint<3> count4(const word (&m)[4]) {
+    word s1, c1, c2;
+    vint<3> r;
+    fa(m[0], m[1], m[2], s1, c1);
+    ha(s1, m[3], r.p[0], c2);
+    ha(c1, c2, r.p[1], r.p[2]);
+    return r;
+}
+
+// x-partner inside 2-cell pairs (cells 2j <-> 2j+1 of a word); valid for both layouts since
+// every word starts at an even x.
+CELLATO_HD word swap_pairs(word v) {
+    constexpr word even = 0x5555555555555555ULL;
+    return ((v & even) << 1) | ((v >> 1) & even);
+}
+
+// ---- evaluator ------------------------------------------------------------------------------
+template <class Dict, class E>
+struct ev;
+
+template <class Dict, class E, class Win>
+CELLATO_HD auto eval(const Win& w) { return ev<Dict, E>::at(w); }
+
+// dictionary index of a state literal, evaluated at compile time outside any function so
+// that device code never references the (host-only) throwing lookup
+template <class Dict, auto V>
+inline constexpr long long index_of = Dict::state_to_index(V);
+
+// per-cell "state index == K" on a P-plane value
+template <long long K>
+struct is_index {
+    template <int N>
+    CELLATO_HD word operator()(const vint<N>& v) const { return eq_const<K>(v); }
+};
+
+template <class Dict, auto V>
+struct ev<Dict, constant<V>> {
+    template <class Win> CELLATO_HD static auto at(const Win&) { return make_const<static_cast<long long>(V)>(); }
+};
+
+template <class Dict, auto V>
+struct ev<Dict, state_constant<V>> {
+    static constexpr long long index = index_of<Dict, V>;
+    template <class Win> CELLATO_HD static auto at(const Win&) { return make_const<index>(); }
+};
+
+template <class Dict, int dx, int dy>
+struct ev<Dict, neighbor_at<dx, dy>> {
+    static_assert(dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1, "neighbour offsets are limited to the 3x3 block");
+    template <class Win> CELLATO_HD static auto at(const Win& w) { return w.template nb<dx, dy>(); }
+};
+
+template <class Dict, class Even, class Odd>
+struct ev<Dict, alternate_algorithms<Even, Odd>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        using A = decltype(as_vint(eval<Dict, Even>(w)));
+        using B = decltype(as_vint(eval<Dict, Odd>(w)));
+        constexpr int N = cmax(A::bits, B::bits);
+        if ((w.t & 1) == 0) return resize<N>(as_vint(eval<Dict, Even>(w)));
+        return resize<N>(as_vint(eval<Dict, Odd>(w)));
+    }
+};
+
+template <class Dict, class C, class A, class B>
+struct ev<Dict, if_then_else<C, A, B>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        const word c = eval<Dict, C>(w);
+        auto a = eval<Dict, A>(w);
+        auto b = eval<Dict, B>(w);
+        if constexpr (std::is_same_v<decltype(a), word> && std::is_same_v<decltype(b), word>)
+            return select(c, a, b);
+        else
+            return select(c, as_vint(a), as_vint(b));
+    }
+};
+
+template <class Dict, class L, class R>
+struct ev<Dict, bit_and_<L, R>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        auto l = eval<Dict, L>(w);
+        auto r = eval<Dict, R>(w);
+        if constexpr (std::is_same_v<decltype(l), word> && std::is_same_v<decltype(r), word>)
+            return word(l & r);
+        else
+            return zip(as_vint(l), as_vint(r), [](word x, word y) { return x & y; });
+    }
+};
+
+template <class Dict, class L, class R>
+struct ev<Dict, bit_or_<L, R>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        auto l = eval<Dict, L>(w);
+        auto r = eval<Dict, R>(w);
+        if constexpr (std::is_same_v<decltype(l), word> && std::is_same_v<decltype(r), word>)
+            return word(l | r);
+        else
+            return zip(as_vint(l), as_vint(r), [](word x, word y) { return x | y; });
+    }
+};
+
+template <class Dict, class L, class R>
+struct ev<Dict, plus<L, R>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) { return add(as_vint(eval<Dict, L>(w)), as_vint(eval<Dict, R>(w))); }
+};
+
+template <class Dict, class L, int M>
+struct ev<Dict, modulo<L, constant<M>>> {
+    static_assert(M > 0 && (M & (M - 1)) == 0, "modulo is supported for powers of two");
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) { return resize<cmax(1, bit_width(M) - 1)>(as_vint(eval<Dict, L>(w))); }
+};
+
+template <class Dict, class V>
+struct ev<Dict, not_<V>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return ~word(eval<Dict, V>(w)); }
+};
+template <class Dict, class L, class R>
+struct ev<Dict, and_<L, R>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return word(eval<Dict, L>(w)) & word(eval<Dict, R>(w)); }
+};
+template <class Dict, class L, class R>
+struct ev<Dict, or_<L, R>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return word(eval<Dict, L>(w)) | word(eval<Dict, R>(w)); }
+};
+
+template <class Dict, class L, class R>
+struct ev<Dict, equals<L, R>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) { return eq(as_vint(eval<Dict, L>(w)), as_vint(eval<Dict, R>(w))); }
+};
+// equality of a neighbour with a state literal: per-cell test before the shift
+template <class Dict, int dx, int dy, auto V>
+struct ev<Dict, equals<neighbor_at<dx, dy>, state_constant<V>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        return w.template nbf<dx, dy>(is_index<index_of<Dict, V>>{});
+    }
+};
+template <class Dict, class L, class R>
+struct ev<Dict, not_equals<L, R>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return ~eq(as_vint(eval<Dict, L>(w)), as_vint(eval<Dict, R>(w))); }
+};
+template <class Dict, class L, class R>
+struct ev<Dict, greater_than<L, R>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return gt(as_vint(eval<Dict, L>(w)), as_vint(eval<Dict, R>(w))); }
+};
+template <class Dict, class L, class R>
+struct ev<Dict, less_than<L, R>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return gt(as_vint(eval<Dict, R>(w)), as_vint(eval<Dict, L>(w))); }
+};
+
+template <class Dict, class V, int b>
+struct ev<Dict, has_bit_set<V, b>> {
+    template <class Win> CELLATO_HD static word at(const Win& w) { return plane(as_vint(eval<Dict, V>(w)), b); }
+};
+// a bit of a neighbour: shift only that plane
+template <class Dict, int dx, int dy, int b>
+struct ev<Dict, has_bit_set<neighbor_at<dx, dy>, b>> {
+    struct bit_of { template <int N> CELLATO_HD word operator()(const vint<N>& v) const { return plane(v, b); } };
+    template <class Win> CELLATO_HD static word at(const Win& w) { return w.template nbf<dx, dy>(bit_of{}); }
+};
+
+// ---- neighbourhood counts -------------------------------------------------------------------
+// Masks of "neighbour (dx, dy) matches the target", in the 8 Moore positions.
+template <class Dict, class C>
+struct moore_masks {
+    template <class Win>
+    CELLATO_HD static void get(const Win& w, word (&m)[8]) {
+        if constexpr (is_state_literal<C>::value) {
+            constexpr long long k = index_of<Dict, C::value>;
+            const is_index<k> f{};
+            m[0] = w.template nbf<-1, -1>(f); m[1] = w.template nbf<0, -1>(f); m[2] = w.template nbf<1, -1>(f);
+            m[3] = w.template nbf<-1, 0>(f);                                   m[4] = w.template nbf<1, 0>(f);
+            m[5] = w.template nbf<-1, 1>(f);  m[6] = w.template nbf<0, 1>(f);  m[7] = w.template nbf<1, 1>(f);
+        } else {
+            const auto target = as_vint(eval<Dict, C>(w));
+            m[0] = eq(w.template nb<-1, -1>(), target); m[1] = eq(w.template nb<0, -1>(), target);
+            m[2] = eq(w.template nb<1, -1>(), target);  m[3] = eq(w.template nb<-1, 0>(), target);
+            m[4] = eq(w.template nb<1, 0>(), target);   m[5] = eq(w.template nb<-1, 1>(), target);
+            m[6] = eq(w.template nb<0, 1>(), target);   m[7] = eq(w.template nb<1, 1>(), target);
+        }
+    }
+};
+
+template <class Dict, class C>
+struct vn_masks {
+    template <class Win>
+    CELLATO_HD static void get(const Win& w, word (&m)[4]) {
+        if constexpr (is_state_literal<C>::value) {
+            constexpr long long k = index_of<Dict, C::value>;
+            const is_index<k> f{};
+            m[0] = w.template nbf<0, 1>(f); m[1] = w.template nbf<0, -1>(f);
+            m[2] = w.template nbf<1, 0>(f); m[3] = w.template nbf<-1, 0>(f);
+        } else {
+            const auto target = as_vint(eval<Dict, C>(w));
+            m[0] = eq(w.template nb<0, 1>(), target); m[1] = eq(w.template nb<0, -1>(), target);
+            m[2] = eq(w.template nb<1, 0>(), target); m[3] = eq(w.template nb<-1, 0>(), target);
+        }
+    }
+};
+
+template <class Dict, class C>
+struct ev<Dict, count_neighbors<C, moore_8_neighbors>> {
+    template <class Win>
+    CELLATO_HD static vint<4> at(const Win& w) {
+        word m[8];
+        moore_masks<Dict, C>::get(w, m);
+        return count8(m);
+    }
+};
+
+template <class Dict, class C>
+struct ev<Dict, count_neighbors<C, von_neumann_4_neighbors>> {
+    template <class Win>
+    CELLATO_HD static vint<3> at(const Win& w) {
+        word m[4];
+        vn_masks<Dict, C>::get(w, m);
+        return count4(m);
+    }
+};
+
+// "any neighbour matches": count > 0 without building the count
+template <class Dict, class C, auto Z>
+struct ev<Dict, greater_than<count_neighbors<C, moore_8_neighbors>, constant<Z>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        if constexpr (Z == 0) {
+            word m[8];
+            moore_masks<Dict, C>::get(w, m);
+            return m[0] | m[1] | m[2] | m[3] | m[4] | m[5] | m[6] | m[7];
+        } else {
+            return gt(ev<Dict, count_neighbors<C, moore_8_neighbors>>::at(w), make_const<Z>());
+        }
+    }
+};
+template <class Dict, class C, auto Z>
+struct ev<Dict, greater_than<count_neighbors<C, von_neumann_4_neighbors>, constant<Z>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        if constexpr (Z == 0) {
+            word m[4];
+            vn_masks<Dict, C>::get(w, m);
+            return m[0] | m[1] | m[2] | m[3];
+        } else {
+            return gt(ev<Dict, count_neighbors<C, von_neumann_4_neighbors>>::at(w), make_const<Z>());
+        }
+    }
+};
+
+// ---- Margolus neighbourhood ---------------------------------------------------------------
+// In global coordinates: on even steps the 2x2 blocks start at even (x, y), on odd steps at
+// odd (x, y) (standard.hpp:185-248). The vertical partner row and the horizontal partner
+// column are chosen per cell from the row parity mask and the cell's x parity.
+constexpr word even_cols = 0x5555555555555555ULL;   // cells with even x
+constexpr word odd_cols = 0xAAAAAAAAAAAAAAAAULL;    // cells with odd x
+
+// value of the vertical block partner (dy = +-1), for any per-cell function F
+template <int dx, class Win, class F>
+CELLATO_HD word vpartner_f(const Win& w, F f) {
+    const word up = w.template nbf<dx, -1>(f), down = w.template nbf<dx, 1>(f);
+    // even step: even rows pair with the row below; odd step: with the row above
+    return (w.t & 1) == 0 ? select(w.yodd(), up, down) : select(w.yodd(), down, up);
+}
+
+template <class Dict, auto V>
+struct ev<Dict, count_neighbors<state_constant<V>, margolus_alternating_neighborhood>> {
+    template <class Win>
+    CELLATO_HD static vint<3> at(const Win& w) {
+        const is_index<index_of<Dict, V>> f{};
+        const word own = w.template nbf<0, 0>(f);
+        const word ver = vpartner_f<0>(w, f);
+        word own_h, ver_h;   // the same two rows, at the horizontal block partner
+        if ((w.t & 1) == 0) {
+            own_h = swap_pairs(own);
+            ver_h = swap_pairs(ver);
+        } else {
+            own_h = (w.template nbf<-1, 0>(f) & even_cols) | (w.template nbf<1, 0>(f) & odd_cols);
+            ver_h = (vpartner_f<-1>(w, f) & even_cols) | (vpartner_f<1>(w, f) & odd_cols);
+        }
+        const word m[4] = {own, ver, own_h, ver_h};
+        return count4(m);
+    }
+};
+
+template <class Dict>
+struct ev<Dict, margolus_180_neighbor> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        const word yo = w.yodd();
+        if ((w.t & 1) == 0) {
+            auto v = select(yo, w.template nb<0, -1>(), w.template nb<0, 1>());
+            decltype(v) r;
+#pragma unroll
+            for (int i = 0; i < decltype(v)::bits; ++i) r.p[i] = swap_pairs(v.p[i]);
+            return r;
+        }
+        auto left = select(yo, w.template nb<-1, 1>(), w.template nb<-1, -1>());
+        auto right = select(yo, w.template nb<1, 1>(), w.template nb<1, -1>());
+        return zip(left, right, [](word l, word r) { return (l & even_cols) | (r & odd_cols); });
+    }
+};
+
+// The reference's evaluator name: evaluator<word, dictionary, Expression>
+// (evaluators/bit_planes.hpp:48-49). Only 64-bit words are implemented on the GPU.
+template <typename cell_row_type, typename state_dictionary_type, typename Expression>
+struct evaluator {
+    static_assert(std::is_same_v<cell_row_type, word>, "the HIP traversers use 64-bit words");
+    using dictionary = state_dictionary_type;
+    using expression = Expression;
+    static constexpr int planes = state_dictionary_type::needed_bits;
+
+    template <class Win>
+    CELLATO_HD static vint<planes> evaluate(const Win& w) {
+        return resize<planes>(as_vint(eval<state_dictionary_type, Expression>(w)));
+    }
+};
+
+} // namespace cellato::evaluators::bit_planes
+
+#endif // CELLATO_EVALUATORS_BIT_PLANES_HPP

@@ -1,0 +1,73 @@
+"""The C-ABI library loads, exports every symbol include/cellato_hip.h declares, reports
+the same rule registry as the oracle, and rejects invalid arguments before touching the GPU."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+import oracle_lib as O
+from cellato_amd import _lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "cellato_hip.h"
+
+
+def declared():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(cellato_\w+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_registry_matches_oracle():
+    L = _lib.lib()
+    assert L.cellato_rule_count() == O.lib().oracle_rule_count() == 10
+    for name, i in O.rule_ids().items():
+        assert L.cellato_rule_name(i).decode() == name
+        assert L.cellato_rule_find(name.encode()) == i
+        assert L.cellato_rule_planes(i) == O.planes(name)
+        buf = (ctypes.c_int32 * 32)()
+        n = L.cellato_rule_states(i, buf, 32)
+        assert list(buf[:n]) == list(O.states(name))
+    assert L.cellato_rule_find(b"no-such-rule") == _lib.E_RULE
+    assert L.cellato_rule_name(10) is None
+    assert L.cellato_max_temporal_steps() >= 8
+
+
+FAKE = [0x1000 * (i + 1) for i in range(5)]   # never dereferenced: every case fails validation
+
+
+@pytest.mark.parametrize("args,code", [
+    (dict(rule=11), _lib.E_RULE),
+    (dict(layout=7), _lib.E_LAYOUT),
+    (dict(width=100), _lib.E_GEOMETRY),
+    (dict(width=0), _lib.E_GEOMETRY),
+    (dict(layout=1, width=64, height=12), _lib.E_GEOMETRY),
+    (dict(steps=-1), _lib.E_ARG),
+    (dict(a=[0]), _lib.E_ARG),
+])
+def test_invalid_arguments_are_rejected(args, code):
+    p = dict(rule=0, layout=0, a=FAKE, b=FAKE, width=128, height=16, steps=3)
+    p.update(args)
+    with pytest.raises(_lib.CellatoError) as e:
+        _lib.simple(p["rule"], p["layout"], p["a"], p["b"], p["width"], p["height"], 0, p["steps"])
+    assert e.value.code == code
+    assert _lib.lib().cellato_hip_last_error().decode()
+
+
+def test_temporal_depth_is_checked():
+    for T in (0, 9, -2):
+        with pytest.raises(_lib.CellatoError) as e:
+            _lib.temporal(0, 0, FAKE, FAKE, 128, 16, 0, 4, T)
+        assert e.value.code == _lib.E_TEMPORAL
+
+
+def test_zero_steps_launch_nothing():
+    assert _lib.simple(0, 0, FAKE, FAKE, 128, 16, 0, 0) is False
+    assert _lib.temporal(9, 1, FAKE, FAKE, 64, 16, 0, 0, 4) is False

@@ -1,0 +1,71 @@
+"""Bit-exactness of the HIP traversers against the CPU oracle (standard evaluator), for all
+ten automata, both layouts, the simple traverser and the temporal traverser at several
+depths, over grid shapes that exercise every lane mapping (rows narrower than a wave, exactly
+one wave, several full waves, a partial last wave), odd start steps, steps that are not a
+multiple of the depth, and very short tori."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import run_gpu
+
+pytestmark = pytest.mark.gpu
+
+RULES = ["game-of-life", "wire", "forest-fire", "excitable", "brian", "maze", "traffic", "fluid", "critters", "cyclic"]
+LINEAR_SHAPES = [(192, 70), (4096, 40), (8192, 50), (8320, 33), (128, 1), (64, 3)]
+TILED_SHAPES = [(40, 24), (512, 16), (1024, 24), (1040, 32), (8, 8)]
+TRAVERSERS = [("simple", 1), ("temporal", 1), ("temporal", 2), ("temporal", 3), ("temporal", 5), ("temporal", 8)]
+
+
+def iid(rule, w, h, seed):
+    rng = np.random.default_rng(seed)
+    st = O.states(rule)
+    return st[rng.integers(0, len(st), size=(h, w))].astype(np.int32)
+
+
+@pytest.mark.parametrize("layout", [0, 1], ids=["linear", "tiled"])
+@pytest.mark.parametrize("rule", RULES)
+def test_matches_standard_evaluator(rule, layout):
+    shapes = LINEAR_SHAPES if layout == 0 else TILED_SHAPES
+    for si, (w, h) in enumerate(shapes):
+        v = iid(rule, w, h, 100 + si)
+        for t0 in (0, 3):
+            want = O.run_standard(rule, v, 11, t0=t0)
+            for kind, T in TRAVERSERS:
+                got = run_gpu(rule, v, layout, kind, T, 11, t0)
+                assert np.array_equal(got, want), f"{rule} {w}x{h} {kind} T={T} t0={t0}"
+
+
+@pytest.mark.parametrize("rule", ["game-of-life", "critters", "cyclic", "traffic"])
+def test_chunk_sizes(rule):
+    v = iid(rule, 8320, 45, 9)
+    want = O.run_standard(rule, v, 16, t0=1)
+    for chunk in (1, 2, 3, 7, 64):
+        for kind, T in [("simple", 1), ("temporal", 4), ("temporal", 8)]:
+            assert np.array_equal(run_gpu(rule, v, 0, kind, T, 16, 1, chunk), want), (chunk, kind, T)
+
+
+@pytest.mark.parametrize("rule", RULES)
+def test_2048_subset_run(rule):
+    # the bit-exactness subset of the 16384^2 benchmark: 2048^2, against the CPU bit_planes
+    # oracle (itself equal to the standard evaluator, see test_oracle_pinning)
+    import cellato_amd
+    v = cellato_amd.iid_values(rule, 2048, 2048, 5)
+    want = O.run_bitplanes(rule, v, 96)
+    for layout, kind, T in [(0, "simple", 1), (0, "temporal", 4), (0, "temporal", 8), (1, "simple", 1), (1, "temporal", 6)]:
+        assert np.array_equal(run_gpu(rule, v, layout, kind, T, 96), want), (layout, kind, T)
+
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_checksums.json").read_text())
+
+
+@pytest.mark.parametrize("rule", RULES)
+def test_published_checksum_16384(rule):
+    g = next(e for e in GOLD if e["automaton"] == rule and e["x_size"] == e["y_size"] == 16384 and e["steps"] == 128)
+    init = O.init_reference(rule, 16384, 16384, g["seed"])
+    for layout, kind, T in [(0, "simple", 1), (0, "temporal", 8), (1, "temporal", 4)]:
+        out = run_gpu(rule, init, layout, kind, T, 128)
+        assert O.checksum(out) == g["checksum"], (g["source"], layout, kind, T)
