@@ -14,11 +14,10 @@ namespace cellato::hip {
 constexpr int MAX_T = 8;   // generations per launch of the temporal traverser
 
 enum simple_variant : int {
-    SIMPLE_SEGMENT = 0,    // rows of 64k columns: DPP, end lanes load the outside column
-    SIMPLE_SEGMENT_ANY = 1,// rows of other widths > 64: bpermute, same halo loads
-    SIMPLE_RING64 = 2,     // rows of exactly 64 columns
-    SIMPLE_GROUPS = 3,     // rows of fewer than 64 columns, several rows per wave
-    SIMPLE_VARIANTS = 4
+    SIMPLE_SEGMENT = 0,    // rows wider than 64 columns: DPP, end lanes load the outside column
+    SIMPLE_RING64 = 1,     // rows of exactly 64 columns
+    SIMPLE_GROUPS = 2,     // rows of fewer than 64 columns, several rows per wave
+    SIMPLE_VARIANTS = 3
 };
 
 enum temporal_variant : int {
@@ -40,7 +39,6 @@ struct rule_kernels {
 template <class Eval, int L>
 constexpr void fill_layout(rule_kernels& k) {
     k.simple[L][SIMPLE_SEGMENT] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_SEGMENT, HALO_LOAD>);
-    k.simple[L][SIMPLE_SEGMENT_ANY] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_PERMUTE, HALO_LOAD>);
     k.simple[L][SIMPLE_RING64] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_RING64, HALO_NONE>);
     k.simple[L][SIMPLE_GROUPS] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_PERMUTE, HALO_NONE>);
     k.temporal[L][TEMPORAL_SEGMENT][0] = nullptr;

@@ -55,7 +55,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         a.groups = 1;
         a.stride = 64;
         a.nseg = (wx + 63) / 64;
-        pl.fn = k.simple[layout][wx % 64 == 0 ? SIMPLE_SEGMENT : SIMPLE_SEGMENT_ANY];
+        pl.fn = k.simple[layout][SIMPLE_SEGMENT];
     } else {
         a.n = 64;
         a.groups = 1;

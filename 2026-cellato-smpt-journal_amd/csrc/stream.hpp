@@ -240,39 +240,35 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     const bool lane_used = g < a.groups;
     int col;                  // this lane's word column
     bool owner;               // whether this lane stores its column
-    int halo_col = 0;         // the outside column for an end lane (HALO_LOAD)
+    int halo_col = 0;         // the outside column of an end lane (HALO_LOAD)
+    bool end_lane = false;
     xch<X> xc{};
     if constexpr (HALO == HALO_LANES) {
+        // segments overlap by two columns; lanes 0 and 63 are halo lanes
         const int first = seg * a.stride - 1;
         col = wrap(first + lane, a.wx);
         const int owned = seg * a.stride + lane - 1;
         owner = lane >= 1 && lane <= 62 && owned < a.wx;
+    } else if constexpr (HALO == HALO_LOAD) {
+        // full 64-column segments; the last one is moved back to end at the row's end
+        // (columns it shares with its neighbour are written twice with the same value)
+        const int first = seg * a.stride < a.wx - 64 ? seg * a.stride : a.wx - 64;
+        col = first + lane;
+        owner = true;
+        end_lane = lane == 0 || lane == 63;
+        halo_col = lane == 0 ? wrap(first - 1, a.wx) : wrap(first + 64, a.wx);
     } else {
-        const int first = seg * a.stride;
-        col = first + w;
-        owner = lane_used && col < a.wx;
-        if (col >= a.wx) col = a.wx - 1;           // idle lane: read something valid
-        if constexpr (LOADS_HALO) {
-            const int last = (a.wx - first < a.n ? a.wx - first : a.n) - 1;
-            halo_col = w == 0 ? wrap(first - 1, a.wx) : wrap(first + last + 1, a.wx);
-        }
+        // whole rows: groups of n lanes, one row each, wrapping inside the group
+        col = w < a.wx ? w : a.wx - 1;
+        owner = lane_used;
     }
     if constexpr (X == XCH_PERMUTE) {
+        static_assert(HALO == HALO_NONE, "permuted exchange is used for whole-row groups only");
         const int base = g * a.n;
-        if constexpr (HALO == HALO_NONE) {
-            xc.src_l = 4 * (base + (w == 0 ? a.n - 1 : w - 1));
-            xc.src_r = 4 * (base + (w == a.n - 1 ? 0 : w + 1));
-            xc.halo_l = xc.halo_r = false;
-        } else {
-            const int first = seg * a.stride;
-            const int last = (a.wx - first < a.n ? a.wx - first : a.n) - 1;
-            xc.src_l = 4 * (w == 0 ? 0 : lane - 1);
-            xc.src_r = 4 * (w >= last ? lane : lane + 1);
-            xc.halo_l = w == 0;
-            xc.halo_r = w == last;
-        }
+        xc.src_l = 4 * (base + (w == 0 ? a.n - 1 : w - 1));
+        xc.src_r = 4 * (base + (w == a.n - 1 ? 0 : w + 1));
+        xc.halo_l = xc.halo_r = false;
     }
-    const bool end_lane = LOADS_HALO && (w == 0 || w == ((a.wx - seg * a.stride < a.n ? a.wx - seg * a.stride : a.n) - 1));
 
     // rows handled by this lane's group
     const int r0 = (chunk * a.groups + (lane_used ? g : 0)) * a.chunk;
@@ -318,7 +314,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
         for (int p = 0; p < P; ++p) hpf[u][p] = 0;
         load_row(pf[u], hpf[u]);
     }
-    int ycur = wrap(r0 - T - 1, a.h);   // wrapped row s - 1: the row stage 0 computes
+    int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
 
     for (int ib = 0; ib < nsteps; ib += PREFETCH) {
 #pragma unroll

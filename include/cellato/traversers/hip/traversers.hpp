@@ -1,0 +1,126 @@
+// The HIP traversers as C++ templates with the reference's interface
+// (include/traversers/cuda/simple.hpp:25-106, temporal.hpp:24-150):
+//
+//   traverser<evaluator, grid> t;  t.init(grid, params);  t.run(steps[, callback]);
+//   grid result = t.fetch_result();
+//
+// Each call goes through the C ABI of include/cellato_hip.h (libcellato_hip.so), which holds
+// the gfx950 kernels of the automata registered in include/cellato/automata/all.hpp; the
+// evaluator type selects the rule. Errors surface as the reference's exception types:
+// std::runtime_error for invalid parameters, cellato::memory::grids::hip_error for HIP.
+#ifndef CELLATO_TRAVERSERS_HIP_TRAVERSERS_HPP
+#define CELLATO_TRAVERSERS_HIP_TRAVERSERS_HPP
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "../../../cellato_hip.h"
+#include "../../automata/all.hpp"
+#include "../../experiments/run_params.hpp"
+#include "../../memory/bit_planes_grid.hpp"
+
+namespace cellato::traversers::hip {
+
+// rule id (C ABI) of an evaluator's expression; fails to compile for an unregistered rule
+template <class Expression>
+constexpr int rule_id_of() {
+    int id = -1;
+    [&]<int... ids>(std::integer_sequence<int, ids...>) {
+        ((id = std::is_same_v<Expression, typename cellato::automata::by_id<ids>::config::algorithm> ? ids : id), ...);
+    }(std::make_integer_sequence<int, cellato::automata::RULE_COUNT>{});
+    return id;
+}
+
+inline void check(int rc) {
+    if (rc == CELLATO_E_HIP) throw cellato::memory::grids::hip_error(cellato_hip_last_error());
+    if (rc < 0) throw std::runtime_error(cellato_hip_last_error());
+}
+
+template <typename evaluator_type, typename grid_type, bool Temporal>
+class traverser_base {
+    using evaluator_t = evaluator_type;
+    using grid_t = grid_type;
+    static constexpr int P = grid_t::needed_bits;
+    static constexpr int rule = rule_id_of<typename evaluator_t::expression>();
+    static_assert(rule >= 0, "this rule is not compiled into libcellato_hip.so (include/cellato/automata/all.hpp)");
+    using device_grid_t = std::invoke_result_t<decltype(&grid_t::to_hip), grid_t>;
+
+public:
+    static constexpr bool is_CUDA = true;   // the reference's name for "runs on the GPU"
+
+    struct no_callback {};
+
+    void init(grid_t grid, const cellato::run::run_params& params) {
+        if (params.word_size != 0 && params.word_size != 64) throw std::runtime_error("the HIP traversers use 64-bit words");
+        if constexpr (Temporal) {
+            if (params.temporal_steps < 1 || params.temporal_steps > cellato_max_temporal_steps())
+                throw std::runtime_error("Invalid temporal steps for the temporal traverser.");
+        }
+        _params = params;
+        _cur = grid.to_hip();
+        _other = grid.to_hip();
+        _t = 0;
+        _ready = true;
+    }
+
+    template <typename callback = no_callback>
+    void run(int steps, callback&& cb = no_callback{}) {
+        if (!_ready) throw std::runtime_error("run() before init()");
+        if constexpr (std::is_same_v<std::decay_t<callback>, no_callback>) {
+            advance(steps);
+        } else {
+            auto host = _cur.to_cpu();
+            cb(0, host);
+            for (int k = 0; k < steps; ++k) {
+                advance(1);
+                auto h = _cur.to_cpu();
+                cb(k + 1, h);
+            }
+        }
+    }
+
+    grid_t fetch_result() {
+        cellato::memory::grids::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        grid_t r = _cur.to_cpu();
+        _cur.free_device_memory();
+        _other.free_device_memory();
+        _ready = false;
+        return r;
+    }
+
+private:
+    void advance(int steps) {
+        auto a = _cur.data();
+        auto b = _other.data();
+        const int w = static_cast<int>(_cur.x_size_original()), h = static_cast<int>(_cur.y_size_original());
+        int in_b = 0;
+        if constexpr (Temporal)
+            check(cellato_hip_temporal(rule, grid_t::layout, a.data(), b.data(), w, h, _t, steps, _params.temporal_steps,
+                                       _params.chunk_rows, nullptr, &in_b));
+        else
+            check(cellato_hip_simple(rule, grid_t::layout, a.data(), b.data(), w, h, _t, steps, _params.chunk_rows,
+                                     nullptr, &in_b));
+        if (in_b) std::swap(_cur, _other);
+        _t += steps;
+    }
+
+    cellato::run::run_params _params;
+    device_grid_t _cur, _other;
+    int _t = 0;
+    bool _ready = false;
+};
+
+namespace simple {
+template <typename evaluator_type, typename grid_type>
+using traverser = traverser_base<evaluator_type, grid_type, false>;
+}
+namespace temporal {
+template <typename evaluator_type, typename grid_type>
+using traverser = traverser_base<evaluator_type, grid_type, true>;
+}
+
+} // namespace cellato::traversers::hip
+
+#endif

@@ -1,0 +1,116 @@
+// CPU run of the product's bit-plane evaluator (include/cellato/evaluators/bit_planes.hpp)
+// through a window that reads a whole grid from host memory: the same evaluator code the
+// gfx950 kernels inline, checked against the oracle's standard evaluator for every rule,
+// both layouts and both time-step parities. No GPU needed.
+#include <cstdint>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "cellato/automata/all.hpp"
+#include "cellato/evaluators/bit_planes.hpp"
+#include "layouts.hpp"
+#include "standard.hpp"
+
+using namespace cellato::evaluators::bit_planes;
+using u64 = std::uint64_t;
+
+template <int P, bool Tiled>
+struct host_window {
+    const u64* planes;    // P planes of wx * h words
+    int wx, h;            // words per row, rows of words
+    int x, y;             // word position
+    int t;
+    int yglobal;          // cell row of the word (linear layout)
+
+    u64 at(int p, int X, int Y) const {
+        X = (X % wx + wx) % wx;
+        Y = (Y % h + h) % h;
+        return planes[static_cast<size_t>(p) * wx * h + static_cast<size_t>(Y) * wx + X];
+    }
+    u64 yodd() const { return Tiled ? 0xFF00FF00FF00FF00ULL : ((yglobal & 1) ? ~0ULL : 0ULL); }
+
+    // word of plane p at word column X shifted by dy cell rows
+    template <int dy>
+    u64 vrow(int p, int X) const {
+        if constexpr (!Tiled) return at(p, X, y + dy);
+        else if constexpr (dy == 0) return at(p, X, y);
+        else if constexpr (dy < 0) return (at(p, X, y) << 8) | (at(p, X, y - 1) >> 56);
+        else return (at(p, X, y) >> 8) | (at(p, X, y + 1) << 56);
+    }
+    template <int dx>
+    static u64 hsh(u64 own, u64 side) {
+        if constexpr (dx == 0) return own;
+        if constexpr (!Tiled) {
+            if constexpr (dx < 0) return (own << 1) | (side >> 63);
+            else return (own >> 1) | (side << 63);
+        } else {
+            constexpr u64 C0 = 0x0101010101010101ULL, C7 = 0x8080808080808080ULL;
+            if constexpr (dx < 0) return ((own << 1) & ~C0) | ((side >> 7) & C0);
+            else return ((own >> 1) & ~C7) | ((side << 7) & C7);
+        }
+    }
+    template <int dx, int dy>
+    vint<P> nb() const {
+        vint<P> r;
+        for (int p = 0; p < P; ++p) r.p[p] = hsh<dx>(vrow<dy>(p, x), vrow<dy>(p, x + dx));
+        return r;
+    }
+    template <int dx, int dy, class F>
+    u64 nbf(F f) const {
+        vint<P> o, s;
+        for (int p = 0; p < P; ++p) { o.p[p] = vrow<dy>(p, x); s.p[p] = vrow<dy>(p, x + dx); }
+        return hsh<dx>(f(o), f(s));
+    }
+};
+
+template <class Cfg, bool Tiled>
+bool check(int W, int H, int t0, int steps) {
+    using dict = typename Cfg::state_dictionary;
+    constexpr int P = dict::needed_bits;
+    using E = evaluator<u64, dict, typename Cfg::algorithm>;
+    std::mt19937 rng(7 + t0);
+    std::vector<int32_t> vals(static_cast<size_t>(W) * H), idx(vals.size());
+    for (size_t i = 0; i < vals.size(); ++i) {
+        idx[i] = static_cast<int32_t>(rng() % dict::number_of_values);
+        vals[i] = static_cast<int32_t>(dict::index_to_state(idx[i]));
+    }
+    const int wx = Tiled ? W / 8 : W / 64, h = Tiled ? H / 8 : H;
+    std::vector<u64> a(static_cast<size_t>(P) * wx * h), b(a.size());
+    if (Tiled) oracle::layouts::pack_tiled(idx.data(), W, H, P, a.data());
+    else oracle::layouts::pack_linear(idx.data(), W, H, P, a.data());
+    for (int k = 0; k < steps; ++k) {
+        for (int Y = 0; Y < h; ++Y)
+            for (int X = 0; X < wx; ++X) {
+                host_window<P, Tiled> w{a.data(), wx, h, X, Y, t0 + k, Y};
+                auto r = E::evaluate(w);
+                for (int p = 0; p < P; ++p) b[static_cast<size_t>(p) * wx * h + static_cast<size_t>(Y) * wx + X] = r.p[p];
+            }
+        a.swap(b);
+    }
+    oracle::standard::run<Cfg>(vals.data(), W, H, t0, steps, 1);
+    std::vector<int32_t> got(vals.size());
+    if (Tiled) oracle::layouts::unpack_tiled(a.data(), W, H, P, got.data());
+    else oracle::layouts::unpack_linear(a.data(), W, H, P, got.data());
+    for (size_t i = 0; i < got.size(); ++i)
+        if (static_cast<int32_t>(dict::index_to_state(got[i])) != vals[i]) {
+            std::printf("%s %s t0=%d: first mismatch at cell %zu\n", Cfg::name, Tiled ? "tiled" : "linear", t0, i);
+            return false;
+        }
+    return true;
+}
+
+int main() {
+    bool ok = true;
+    int n = 0;
+    cellato::automata::for_each_id([&]<int id>() {
+        using Cfg = typename cellato::automata::by_id<id>::config;
+        for (int t0 : {0, 1}) {
+            ok &= check<Cfg, false>(192, 22, t0, 5);
+            ok &= check<Cfg, true>(48, 24, t0, 5);
+            n += 2;
+        }
+    }, std::make_integer_sequence<int, cellato::automata::RULE_COUNT>{});
+    std::printf("%d cases: %s\n", n, ok ? "ALL OK" : "FAILED");
+    return ok ? 0 : 1;
+}
