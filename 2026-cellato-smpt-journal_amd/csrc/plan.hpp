@@ -36,6 +36,10 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
     const int wx = layout == CELLATO_LAYOUT_LINEAR ? width / 64 : width / 8;
     const int h = layout == CELLATO_LAYOUT_LINEAR ? height : height / 8;
     pass_args& a = pl.args;
+    if (static_cast<unsigned long long>(wx) * static_cast<unsigned long long>(h) * 8ULL >= (1ULL << 32)) {
+        plan_error() = "a bit plane must be smaller than 4 GiB";
+        return CELLATO_E_GEOMETRY;
+    }
     a.wx = wx;
     a.h = h;
     if (wx < 64) {

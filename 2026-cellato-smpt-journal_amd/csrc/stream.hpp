@@ -26,7 +26,6 @@ using cellato::evaluators::bit_planes::vint;
 constexpr int MAX_PLANES = 5;
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int BLOCK_THREADS = 64 * WAVES_PER_BLOCK;
-constexpr int PREFETCH = 4;   // rows loaded ahead of the row being evaluated
 
 enum layout_kind : int { LINEAR = 0, TILED = 1 };
 
@@ -74,6 +73,12 @@ constexpr int DPP_WAVE_ROL1 = 0x134;   // lane i <- lane i+1, lane 63 <- lane 0
 constexpr int DPP_WAVE_SHR1 = 0x138;   // lane i <- lane i-1
 constexpr int DPP_WAVE_ROR1 = 0x13C;   // lane i <- lane i-1, lane 0 <- lane 63
 
+// lanes without a source read 0 (bound_ctrl set): no `old` register to prepare
+template <int ctrl>
+__device__ __forceinline__ uint32_t dpp0(uint32_t src) {
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(src), ctrl, 0xF, 0xF, true));
+}
+
 template <int ctrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
     // bound_ctrl = false: a lane without a source (lane 0 for shr, 63 for shl) keeps `old`
@@ -90,12 +95,17 @@ template <>
 struct xch<XCH_SEGMENT> {
     __device__ uint32_t left(uint32_t v, uint32_t halo) const { return dpp<DPP_WAVE_SHR1>(halo, v); }
     __device__ uint32_t right(uint32_t v, uint32_t halo) const { return dpp<DPP_WAVE_SHL1>(halo, v); }
+    // segment ends that are halo lanes: nothing outside, zero is as good as anything
+    __device__ uint32_t left0(uint32_t v) const { return dpp0<DPP_WAVE_SHR1>(v); }
+    __device__ uint32_t right0(uint32_t v) const { return dpp0<DPP_WAVE_SHL1>(v); }
 };
 
 template <>
 struct xch<XCH_RING64> {
-    __device__ uint32_t left(uint32_t v, uint32_t) const { return dpp<DPP_WAVE_ROR1>(0u, v); }
-    __device__ uint32_t right(uint32_t v, uint32_t) const { return dpp<DPP_WAVE_ROL1>(0u, v); }
+    __device__ uint32_t left(uint32_t v, uint32_t) const { return dpp0<DPP_WAVE_ROR1>(v); }
+    __device__ uint32_t right(uint32_t v, uint32_t) const { return dpp0<DPP_WAVE_ROL1>(v); }
+    __device__ uint32_t left0(uint32_t v) const { return left(v, 0u); }
+    __device__ uint32_t right0(uint32_t v) const { return right(v, 0u); }
 };
 
 template <>
@@ -110,64 +120,96 @@ struct xch<XCH_PERMUTE> {
         const uint32_t x = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_r, static_cast<int>(v)));
         return halo_r ? halo : x;
     }
+    __device__ uint32_t left0(uint32_t v) const { return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_l, static_cast<int>(v))); }
+    __device__ uint32_t right0(uint32_t v) const { return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_r, static_cast<int>(v))); }
 };
 
 // ---- windows ---------------------------------------------------------------------------------
-// rows[0..2] = the words of rows y-1, y, y+1 in this lane's column; halo[] the same for
-// the column outside the segment end this lane sits on (HALO_LOAD only).
+// top / mid / bot = the words of rows y-1, y, y+1 in this lane's column (P planes each);
+// htop / hmid / hbot the same for the column outside the segment end this lane sits on
+// (only read when HALO is set).
 template <int P>
-struct rows3 {
-    word w[3][P];
+using row_t = word[P];
+
+// A row of the linear layout as held in a ring slot: the P plane words of this lane's column
+// plus the edge halves of the neighbouring columns that the 1-cell shifts need (bits 32..63
+// of the column to the left, bits 0..31 of the column to the right), fetched from the
+// neighbouring lanes once when the row enters the ring.
+template <int P>
+struct lrow {
+    word w[P];
+    uint32_t l[P];   // hi half of the left column's word
+    uint32_t r[P];   // lo half of the right column's word
 };
 
+// the word of a cell row shifted by one cell: dx = -1 gives, at bit i, the cell at x - 1
+template <int dx>
+__device__ __forceinline__ word shift1(word own, uint32_t left_hi, uint32_t right_lo) {
+    if constexpr (dx == 0) {
+        return own;
+    } else if constexpr (dx < 0) {
+        return join(__builtin_amdgcn_alignbit(lo32(own), left_hi, 31), __builtin_amdgcn_alignbit(hi32(own), lo32(own), 31));
+    } else {
+        return join(__builtin_amdgcn_alignbit(hi32(own), lo32(own), 1), __builtin_amdgcn_alignbit(right_lo, hi32(own), 1));
+    }
+}
+
 // Linear layout: a word is 64 cells of one row; cell x at bit x % 64.
-template <int P, int X, bool HALO>
+template <int P>
 struct linear_window {
-    const rows3<P>& c;
-    const rows3<P>& hl;
-    xch<X> x;
+    const lrow<P>& top;
+    const lrow<P>& mid;
+    const lrow<P>& bot;
     word yo;    // all ones when the row is odd
     int t;
 
     __device__ word yodd() const { return yo; }
 
-    // neighbour column dx of a per-lane value (own) and its halo counterpart
-    template <int dx>
-    __device__ word hshift(word own, word halo) const {
-        if constexpr (dx == 0) {
-            return own;
-        } else if constexpr (dx < 0) {
-            const uint32_t l = x.left(hi32(own), HALO ? hi32(halo) : 0u);
-            return (own << 1) | (l >> 31);
-        } else {
-            const uint32_t r = x.right(lo32(own), HALO ? lo32(halo) : 0u);
-            return (own >> 1) | (word(r) << 63);
-        }
+    template <int dy>
+    __device__ const lrow<P>& row() const {
+        if constexpr (dy < 0) return top;
+        else if constexpr (dy == 0) return mid;
+        else return bot;
     }
 
     template <int dx, int dy>
     __device__ vint<P> nb() const {
-        vint<P> r;
+        const lrow<P>& r = row<dy>();
+        vint<P> v;
 #pragma unroll
-        for (int p = 0; p < P; ++p) r.p[p] = hshift<dx>(c.w[dy + 1][p], hl.w[dy + 1][p]);
-        return r;
+        for (int p = 0; p < P; ++p) v.p[p] = shift1<dx>(r.w[p], r.l[p], r.r[p]);
+        return v;
     }
 
+    // F is per cell, so F of the neighbour's edge half is the edge half of F of the neighbour
     template <int dx, int dy, class F>
     __device__ word nbf(F f) const {
-        vint<P> own, halo;
+        const lrow<P>& r = row<dy>();
+        vint<P> own;
 #pragma unroll
-        for (int p = 0; p < P; ++p) { own.p[p] = c.w[dy + 1][p]; halo.p[p] = hl.w[dy + 1][p]; }
-        if constexpr (dx == 0) return f(own);
-        else return hshift<dx>(f(own), HALO ? f(halo) : word(0));
+        for (int p = 0; p < P; ++p) own.p[p] = r.w[p];
+        const word fo = f(own);
+        if constexpr (dx == 0) {
+            return fo;
+        } else {
+            vint<P> side;
+#pragma unroll
+            for (int p = 0; p < P; ++p) side.p[p] = dx < 0 ? join(0u, r.l[p]) : join(r.r[p], 0u);
+            const word fs = f(side);
+            return shift1<dx>(fo, hi32(fs), lo32(fs));
+        }
     }
 };
 
 // Tiled layout: a word is an 8x8 block; cell (x, y) at bit (y % 8) * 8 + x % 8.
 template <int P, int X, bool HALO>
 struct tiled_window {
-    const rows3<P>& c;
-    const rows3<P>& hl;
+    const row_t<P>& top;
+    const row_t<P>& mid;
+    const row_t<P>& bot;
+    const row_t<P>& htop;
+    const row_t<P>& hmid;
+    const row_t<P>& hbot;
     xch<X> x;
     int t;
 
@@ -178,10 +220,10 @@ struct tiled_window {
 
     // the block shifted by one cell row, filled from the block above / below
     template <int dy>
-    __device__ static word vshift(const word (&col)[3][P], int p) {
-        if constexpr (dy == 0) return col[1][p];
-        else if constexpr (dy < 0) return (col[1][p] << 8) | (col[0][p] >> 56);
-        else return (col[1][p] >> 8) | (col[2][p] << 56);
+    __device__ static word vshift(const row_t<P>& t_, const row_t<P>& m_, const row_t<P>& b_, int p) {
+        if constexpr (dy == 0) return m_[p];
+        else if constexpr (dy < 0) return (m_[p] << 8) | (t_[p] >> 56);
+        else return (m_[p] >> 8) | (b_[p] << 56);
     }
 
     template <int dx>
@@ -201,7 +243,8 @@ struct tiled_window {
     __device__ vint<P> nb() const {
         vint<P> r;
 #pragma unroll
-        for (int p = 0; p < P; ++p) r.p[p] = hshift<dx>(vshift<dy>(c.w, p), HALO ? vshift<dy>(hl.w, p) : word(0));
+        for (int p = 0; p < P; ++p)
+            r.p[p] = hshift<dx>(vshift<dy>(top, mid, bot, p), HALO ? vshift<dy>(htop, hmid, hbot, p) : word(0));
         return r;
     }
 
@@ -210,8 +253,8 @@ struct tiled_window {
         vint<P> own, halo;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            own.p[p] = vshift<dy>(c.w, p);
-            halo.p[p] = HALO ? vshift<dy>(hl.w, p) : word(0);
+            own.p[p] = vshift<dy>(top, mid, bot, p);
+            halo.p[p] = HALO ? vshift<dy>(htop, hmid, hbot, p) : word(0);
         }
         if constexpr (dx == 0) return f(own);
         else return hshift<dx>(f(own), HALO ? f(halo) : word(0));
@@ -223,11 +266,34 @@ __device__ __forceinline__ int wrap(int v, int m) {
     return r < 0 ? r + m : r;
 }
 
+// Plane access through buffer resources: the row offset is a scalar (SGPR) operand and the
+// column offset a per-lane constant, so a load or store costs no address arithmetic; an
+// offset past the plane reads 0 instead of faulting. Planes are < 4 GiB (checked on the host).
+constexpr int BUFFER_DWORD3 = 0x00020000;   // raw, untyped 32-bit-element access on gfx9
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const word* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<word*>(p), static_cast<short>(0), static_cast<int>(bytes), BUFFER_DWORD3);
+}
+__device__ __forceinline__ word buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(word, __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(voff), static_cast<int>(soff), 0));
+}
+__device__ __forceinline__ void buf_store(word v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                          r, static_cast<int>(voff), static_cast<int>(soff), 0);
+}
+
+// Rows per unrolled loop body: a multiple of 3 so that the 3-row windows rotate by renaming
+// registers instead of moving them; also the prefetch distance in rows.
+template <int T>
+constexpr int unroll_rows() { return T <= 2 ? 6 : 3; }
+
 // ---- one task: a band of columns x a chunk of rows, T generations --------------------------
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     constexpr int P = Eval::planes;
     constexpr bool LOADS_HALO = HALO == HALO_LOAD;
+    constexpr bool GROUPED = X == XCH_PERMUTE;   // lane groups walk different rows
+    constexpr int U = unroll_rows<T>();
     static_assert(!(HALO == HALO_LOAD && T != 1), "loaded halos are exact for one generation only");
 
     const int lane = lane_id();
@@ -235,7 +301,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     const int chunk = task / a.nseg;
 
     // lane -> (group, column)
-    const int g = lane / a.n;
+    const int g = GROUPED ? lane / a.n : 0;
     const int w = lane - g * a.n;
     const bool lane_used = g < a.groups;
     int col;                  // this lane's word column
@@ -274,68 +340,92 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     const int r0 = (chunk * a.groups + (lane_used ? g : 0)) * a.chunk;
     const int r1 = r0 + a.chunk < a.h ? r0 + a.chunk : a.h;
 
-    rows3<P> win[T];         // win[k]: rows of generation k around the row stage k+1 computes
-    rows3<P> hwin;           // halo column, generation 0 (HALO_LOAD)
+    const uint32_t row_bytes = static_cast<uint32_t>(a.wx) * 8u;
+    const uint32_t plane_bytes = row_bytes * static_cast<uint32_t>(a.h);
+    __amdgpu_buffer_rsrc_t rin[P], rout[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        rin[p] = plane_rsrc(a.in[p], plane_bytes);
+        rout[p] = plane_rsrc(a.out[p], plane_bytes);
+    }
+    const uint32_t col_off = static_cast<uint32_t>(col) * 8u;
+    const uint32_t halo_off = static_cast<uint32_t>(halo_col) * 8u;
+
+    // generation-k rows, a ring of 3 slots per stage (linear rows carry their neighbours'
+    // edge halves; tiled rows exchange whole words when used)
+    lrow<P> lwin[T][3];
+    word win[T][3][P];
+    word hwin[3][P];       // halo column, generation 0 (HALO_LOAD, tiled)
+    word pf[U][P];         // prefetched generation-0 rows
+    word hpf[U][P];
 #pragma unroll
     for (int k = 0; k < T; ++k)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int p = 0; p < P; ++p) win[k].w[i][p] = 0;
+            for (int p = 0; p < P; ++p) {
+                win[k][i][p] = 0;
+                lwin[k][i].w[p] = 0;
+                lwin[k][i].l[p] = 0;
+                lwin[k][i].r[p] = 0;
+            }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int p = 0; p < P; ++p) hwin.w[i][p] = 0;
+        for (int p = 0; p < P; ++p) hwin[i][p] = 0;
 
-    const size_t wx = static_cast<size_t>(a.wx);
     // row of the next load, kept wrapped onto [0, h) incrementally (no division in the loop)
-    int lrow = wrap(r0 - T, a.h);
+    int lrow_next = wrap(r0 - T, a.h);
     auto load_row = [&](word (&dst)[P], word (&hdst)[P]) {
-        const size_t row = static_cast<size_t>(lrow) * wx;
+        const uint32_t roff = static_cast<uint32_t>(lrow_next) * row_bytes;
 #pragma unroll
-        for (int p = 0; p < P; ++p) dst[p] = a.in[p][row + col];
+        for (int p = 0; p < P; ++p) {
+            if constexpr (GROUPED) dst[p] = buf_load(rin[p], roff + col_off, 0u);
+            else dst[p] = buf_load(rin[p], col_off, roff);
+            hdst[p] = 0;
+        }
         if constexpr (LOADS_HALO) {
             if (end_lane) {
 #pragma unroll
-                for (int p = 0; p < P; ++p) hdst[p] = a.in[p][row + halo_col];
+                for (int p = 0; p < P; ++p) hdst[p] = buf_load(rin[p], halo_off, roff);
             }
         }
-        lrow = lrow + 1 == a.h ? 0 : lrow + 1;
+        lrow_next = lrow_next + 1 == a.h ? 0 : lrow_next + 1;
     };
 
     // Step i loads row s = r0 - T + i; every group runs the same number of steps, so the
     // loop is uniform across the wave even when groups walk different rows.
     const int nsteps = a.chunk + 2 * T;
-    word pf[PREFETCH][P];
-    word hpf[PREFETCH][P];
 #pragma unroll
-    for (int u = 0; u < PREFETCH; ++u) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) hpf[u][p] = 0;
-        load_row(pf[u], hpf[u]);
-    }
+    for (int u = 0; u < U; ++u) load_row(pf[u], hpf[u]);
     int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
 
-    for (int ib = 0; ib < nsteps; ib += PREFETCH) {
+    for (int ib = 0; ib < nsteps; ib += U) {
 #pragma unroll
-        for (int u = 0; u < PREFETCH; ++u) {
+        for (int u = 0; u < U; ++u) {
             const int s = r0 - T + ib + u;
+            const int nw = u % 3, tp = (u + 1) % 3, md = (u + 2) % 3;   // slots: newest, y-1, y
             ycur = ycur + 1 == a.h ? 0 : ycur + 1;
-            // push the loaded row into generation 0's window
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                win[0].w[0][p] = win[0].w[1][p];
-                win[0].w[1][p] = win[0].w[2][p];
-                win[0].w[2][p] = pf[u][p];
-                if constexpr (LOADS_HALO) {
-                    hwin.w[0][p] = hwin.w[1][p];
-                    hwin.w[1][p] = hwin.w[2][p];
-                    hwin.w[2][p] = hpf[u][p];
+                if constexpr (LAYOUT == LINEAR) {
+                    const word v = pf[u][p];
+                    lwin[0][nw].w[p] = v;
+                    if constexpr (LOADS_HALO) {
+                        lwin[0][nw].l[p] = xc.left(hi32(v), hi32(hpf[u][p]));
+                        lwin[0][nw].r[p] = xc.right(lo32(v), lo32(hpf[u][p]));
+                    } else {
+                        lwin[0][nw].l[p] = xc.left0(hi32(v));
+                        lwin[0][nw].r[p] = xc.right0(lo32(v));
+                    }
+                } else {
+                    win[0][nw][p] = pf[u][p];
+                    if constexpr (LOADS_HALO) hwin[nw][p] = hpf[u][p];
                 }
             }
             load_row(pf[u], hpf[u]);
 
-            // stages: generation k+1 of row s-1-k from generation k rows s-2-k .. s-k
+            // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k
             vint<P> out;
 #pragma unroll
             for (int k = 0; k < T; ++k) {
@@ -343,27 +433,36 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 if constexpr (LAYOUT == LINEAR) {
                     int y = ycur - k;          // only rules with a Margolus block read it
                     while (y < 0) y += a.h;
-                    const linear_window<P, X, LOADS_HALO> wnd{win[k], hwin, xc, (y & 1) ? ~word(0) : word(0), t};
+                    const linear_window<P> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
                     out = Eval::evaluate(wnd);
                 } else {
-                    const tiled_window<P, X, LOADS_HALO> wnd{win[k], hwin, xc, t};
+                    const tiled_window<P, X, LOADS_HALO> wnd{win[k][tp], win[k][md], win[k][nw],
+                                                             hwin[tp], hwin[md], hwin[nw], xc, t};
                     out = Eval::evaluate(wnd);
                 }
                 if (k + 1 < T) {
+                    const int kn = k + 1 < T ? k + 1 : 0;
 #pragma unroll
                     for (int p = 0; p < P; ++p) {
-                        win[k + 1].w[0][p] = win[k + 1].w[1][p];
-                        win[k + 1].w[1][p] = win[k + 1].w[2][p];
-                        win[k + 1].w[2][p] = out.p[p];
+                        if constexpr (LAYOUT == LINEAR) {
+                            lwin[kn][nw].w[p] = out.p[p];
+                            lwin[kn][nw].l[p] = xc.left0(hi32(out.p[p]));
+                            lwin[kn][nw].r[p] = xc.right0(lo32(out.p[p]));
+                        } else {
+                            win[kn][nw][p] = out.p[p];
+                        }
                     }
                 }
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))
             const int yo = s - T;
             if (yo >= r0 && yo < r1 && owner) {
-                const size_t at = static_cast<size_t>(yo) * wx + col;
+                const uint32_t roff = static_cast<uint32_t>(yo) * row_bytes;
 #pragma unroll
-                for (int p = 0; p < P; ++p) a.out[p][at] = out.p[p];
+                for (int p = 0; p < P; ++p) {
+                    if constexpr (GROUPED) buf_store(out.p[p], rout[p], roff + col_off, 0u);
+                    else buf_store(out.p[p], rout[p], col_off, roff);
+                }
             }
         }
     }
@@ -380,7 +479,7 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
     const int blk = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-    const int task = blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64;
+    const int task = __builtin_amdgcn_readfirstlane(blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
     if (task >= a.ntasks) return;
     stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
 }

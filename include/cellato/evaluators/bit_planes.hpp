@@ -160,13 +160,16 @@ CELLATO_HD vint<cmax(A, B)> select(word m, const vint<A>& a, const vint<B>& b) {
 }
 CELLATO_HD word select(word m, word a, word b) { return (a & m) | (b & ~m); }
 
-// population count of 8 masks as a 4-plane value (carry-save adder tree)
+// Population count of the 8 Moore masks as a 4-plane value (carry-save adder tree). The
+// masks come in row order: m[0..2] = row above (x-1, x, x+1), m[3], m[4] = left and right,
+// m[5..7] = row below. The 3-cell sums of the rows above and below are formed by identical
+// expressions for consecutive output rows, so a streaming caller computes each once.
 CELLATO_HD vint<4> count8(const word (&m)[8]) {
     word s1, c1, s2, c2, s3, c3, c4, s5, c5, c6;
     vint<4> r;
     fa(m[0], m[1], m[2], s1, c1);
-    fa(m[3], m[4], m[5], s2, c2);
-    ha(m[6], m[7], s3, c3);
+    fa(m[5], m[6], m[7], s2, c2);
+    ha(m[3], m[4], s3, c3);
     fa(s1, s2, s3, r.p[0], c4);
     fa(c1, c2, c3, s5, c5);
     ha(s5, c4, r.p[1], c6);
