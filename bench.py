@@ -22,6 +22,20 @@ sys.path.insert(0, str(REPO / "2026-cellato-smpt-journal_amd"))
 sys.path.insert(0, str(REPO / "tests"))
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a
+# plain copy kernel); see profiles/stream_kernel_gol_cyclic_16384_hbm_traffic.md
+TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_16384.json"
+
+
+def measured_traffic(automaton, traverser, T, size):
+    try:
+        data = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    for e in data["entries"]:
+        if (e["automaton"], e["traverser"], e["temporal_steps"], e["size"]) == (automaton, traverser, T, size):
+            return int(e["traffic_bytes"])
+    return None
 
 
 def parse():
@@ -180,7 +194,8 @@ def main():
                    "traverser": g["traverser"], "temporal_steps": g["temporal_steps"]},
         "bit_exact": all(r.get("bit_exact_2048_subset", True) for r in variants),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": measured_traffic(g["automaton"], g["traverser"], g["temporal_steps"], a.size),
                      "kernel": f"stream_kernel ({g['traverser']}, T={g['temporal_steps']})",
                      "bytes_per_launch": 2 * C.rule(g["automaton"]).planes * a.size * a.size // 8},
         "roofline_simple": None if simple_gol is None else {
