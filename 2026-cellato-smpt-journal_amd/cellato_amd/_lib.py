@@ -17,6 +17,13 @@ _P64 = ctypes.POINTER(ctypes.c_uint64)
 _PP64 = ctypes.POINTER(_P64)
 
 
+class BatchGrid(ctypes.Structure):
+    """cellato_batch_grid of include/cellato_hip.h"""
+    _fields_ = [("rule", ctypes.c_int), ("layout", ctypes.c_int), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("steps", ctypes.c_int), ("t0", ctypes.c_int), ("temporal_steps", ctypes.c_int),
+                ("chunk_rows", ctypes.c_int), ("a", ctypes.c_void_p * 5), ("b", ctypes.c_void_p * 5)]
+
+
 class CellatoError(RuntimeError):
     """A failed C-ABI call: `code` is the negative status, the message comes from the library."""
 
@@ -32,6 +39,13 @@ def lib():
     """Load libcellato_hip.so once; raise if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64.so.7. Load it first, so that the library's dependency
+        # on libamdhip64.so.7 resolves to that same runtime: two HIP runtimes in one process
+        # would not share streams or device state.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                               "(make -C 2026-cellato-smpt-journal_amd/csrc)")
@@ -50,6 +64,13 @@ def lib():
         L.cellato_hip_temporal.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, _PP64, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_int)]
+        L.cellato_batch_create.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]
+        L.cellato_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.cellato_batch_result_in_b.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cellato_batch_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_longlong)]
+        L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 

@@ -1,0 +1,209 @@
+// Batches of independent grids (different automata, sizes, step counts, depths) advanced by
+// one hipGraph launch. Host planning: every grid's steps are split into passes of its own
+// depth T (the last pass takes the remainder); pass r of every grid forms round r; in each
+// round the passes are grouped by (rule, T) and each group is one persistent batch_kernel
+// launch over the group's task list. Rounds are ordered; the groups of a round run on
+// separate streams. The whole schedule is captured into one graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cellato_hip.h"
+#include "dispatch.hpp"
+#include "plan.hpp"
+
+namespace cellato::hip {
+namespace {
+
+int bfail(int code, const std::string& m) {
+    last_error() = m;
+    return code;
+}
+
+constexpr int SIDE_STREAMS = 8;
+
+struct launch_rec {
+    const void* fn;
+    int task_begin, task_count;
+    int blocks;
+};
+
+struct batch {
+    std::vector<int> result_in_b;
+    std::vector<std::vector<launch_rec>> rounds;
+    void* dev = nullptr;            // pass table + task lists
+    pass_args* dev_passes = nullptr;
+    batch_task* dev_tasks = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t side[SIDE_STREAMS] = {};
+    hipEvent_t fork = nullptr, join[SIDE_STREAMS] = {};
+    long long tasks = 0;
+    int launches = 0;
+
+    ~batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& s : side) if (s) (void)hipStreamDestroy(s);
+        for (auto& e : join) if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+#define BCHECK(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) return bfail(CELLATO_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
+    std::vector<pass_args> passes;
+    // per round: (rule, T) -> list of (pass index, tasks of that pass)
+    std::vector<std::map<std::pair<int, int>, std::vector<int>>> rounds;
+    B.result_in_b.assign(n, 0);
+    for (int gi = 0; gi < n; ++gi) {
+        const cellato_batch_grid& g = grids[gi];
+        if (g.rule < 0 || g.rule >= 10) return bfail(CELLATO_E_RULE, "grid " + std::to_string(gi) + ": unknown rule");
+        if (g.layout != CELLATO_LAYOUT_LINEAR) return bfail(CELLATO_E_LAYOUT, "batches use the linear layout");
+        if (g.width <= 0 || g.width % 64 || g.height <= 0)
+            return bfail(CELLATO_E_GEOMETRY, "grid " + std::to_string(gi) + ": width must be a positive multiple of 64");
+        if (g.steps < 0) return bfail(CELLATO_E_ARG, "negative steps");
+        if (g.temporal_steps < 1 || g.temporal_steps > MAX_T)
+            return bfail(CELLATO_E_TEMPORAL, "grid " + std::to_string(gi) + ": temporal_steps out of range");
+        const int P = kernels_of_rule(g.rule).planes;
+        for (int p = 0; p < P; ++p)
+            if (!g.a[p] || !g.b[p]) return bfail(CELLATO_E_ARG, "null plane pointer");
+        int done = 0, r = 0;
+        while (done < g.steps) {
+            const int T = std::min(g.temporal_steps, g.steps - done);
+            pass_plan pl;
+            const int rc = plan_pass(g.rule, g.layout, g.width, g.height, T, true, g.chunk_rows, pl);
+            if (rc != CELLATO_OK) return bfail(rc, plan_error());
+            for (int p = 0; p < P; ++p) {
+                pl.args.in[p] = r % 2 == 0 ? g.a[p] : g.b[p];
+                pl.args.out[p] = r % 2 == 0 ? g.b[p] : g.a[p];
+            }
+            pl.args.t0 = g.t0 + done;
+            if (static_cast<int>(rounds.size()) <= r) rounds.resize(r + 1);
+            rounds[r][{g.rule, T}].push_back(static_cast<int>(passes.size()));
+            passes.push_back(pl.args);
+            done += T;
+            ++r;
+        }
+        B.result_in_b[gi] = r % 2;
+    }
+
+    std::vector<batch_task> tasks;
+    B.rounds.resize(rounds.size());
+    for (size_t r = 0; r < rounds.size(); ++r) {
+        for (auto& [key, plist] : rounds[r]) {
+            launch_rec L;
+            L.fn = kernels_of_rule(key.first).batch[key.second];
+            L.task_begin = static_cast<int>(tasks.size());
+            for (int pi : plist)
+                for (int t = 0; t < passes[pi].ntasks; ++t) tasks.push_back({pi, t});
+            L.task_count = static_cast<int>(tasks.size()) - L.task_begin;
+            const int waves = std::min(L.task_count, max_waves);
+            L.blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+            B.rounds[r].push_back(L);
+            ++B.launches;
+        }
+    }
+    B.tasks = static_cast<long long>(tasks.size());
+
+    const size_t pbytes = passes.size() * sizeof(pass_args);
+    const size_t tbytes = tasks.size() * sizeof(batch_task);
+    BCHECK(hipMalloc(&B.dev, pbytes + tbytes + 256));
+    B.dev_passes = static_cast<pass_args*>(B.dev);
+    B.dev_tasks = reinterpret_cast<batch_task*>(static_cast<char*>(B.dev) + ((pbytes + 255) / 256) * 256);
+    if (pbytes) BCHECK(hipMemcpy(B.dev_passes, passes.data(), pbytes, hipMemcpyHostToDevice));
+    if (tbytes) BCHECK(hipMemcpy(B.dev_tasks, tasks.data(), tbytes, hipMemcpyHostToDevice));
+
+    // capture the schedule into one graph
+    for (auto& s : B.side) BCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto& e : B.join) BCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    BCHECK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+    hipStream_t origin;
+    BCHECK(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
+    BCHECK(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+    int status = CELLATO_OK;
+    for (auto& round : B.rounds) {
+        if (hipEventRecord(B.fork, origin) != hipSuccess) { status = CELLATO_E_HIP; break; }
+        for (size_t j = 0; j < round.size() && status == CELLATO_OK; ++j) {
+            const launch_rec& L = round[j];
+            hipStream_t s = B.side[j % SIDE_STREAMS];
+            if (j < SIDE_STREAMS && hipStreamWaitEvent(s, B.fork, 0) != hipSuccess) { status = CELLATO_E_HIP; break; }
+            batch_args args{B.dev_passes, B.dev_tasks + L.task_begin, L.task_count};
+            void* kargs[] = {&args};
+            if (hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, s) != hipSuccess) status = CELLATO_E_HIP;
+        }
+        for (size_t j = 0; j < std::min(round.size(), static_cast<size_t>(SIDE_STREAMS)) && status == CELLATO_OK; ++j) {
+            if (hipEventRecord(B.join[j], B.side[j]) != hipSuccess || hipStreamWaitEvent(origin, B.join[j], 0) != hipSuccess)
+                status = CELLATO_E_HIP;
+        }
+        if (status != CELLATO_OK) break;
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(origin, &graph);
+    (void)hipStreamDestroy(origin);
+    if (status != CELLATO_OK || end != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return bfail(CELLATO_E_HIP, "graph capture of the batch failed");
+    }
+    B.graph = graph;
+    BCHECK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    return CELLATO_OK;
+}
+
+} // namespace
+} // namespace cellato::hip
+
+using namespace cellato::hip;
+
+extern "C" {
+
+int cellato_batch_create(const cellato_batch_grid* grids, int n, int max_waves, void** handle) {
+    if (!handle || (n > 0 && !grids) || n < 0) return bfail(CELLATO_E_ARG, "null argument");
+    *handle = nullptr;
+    auto B = std::make_unique<batch>();
+    const int rc = build(*B, grids, n, max_waves > 0 ? max_waves : 8192);
+    if (rc != CELLATO_OK) return rc;
+    *handle = B.release();
+    return CELLATO_OK;
+}
+
+int cellato_batch_launch(void* handle, void* stream) {
+    if (!handle) return bfail(CELLATO_E_ARG, "null batch");
+    auto* B = static_cast<batch*>(handle);
+    const hipError_t e = hipGraphLaunch(B->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return bfail(CELLATO_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return CELLATO_OK;
+}
+
+int cellato_batch_result_in_b(void* handle, int grid) {
+    if (!handle) return bfail(CELLATO_E_ARG, "null batch");
+    auto* B = static_cast<batch*>(handle);
+    if (grid < 0 || grid >= static_cast<int>(B->result_in_b.size())) return bfail(CELLATO_E_ARG, "grid index out of range");
+    return B->result_in_b[grid];
+}
+
+int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tasks) {
+    if (!handle) return bfail(CELLATO_E_ARG, "null batch");
+    auto* B = static_cast<batch*>(handle);
+    if (rounds) *rounds = static_cast<int>(B->rounds.size());
+    if (launches) *launches = B->launches;
+    if (tasks) *tasks = B->tasks;
+    return CELLATO_OK;
+}
+
+int cellato_batch_destroy(void* handle) {
+    delete static_cast<batch*>(handle);
+    return CELLATO_OK;
+}
+
+} // extern "C"

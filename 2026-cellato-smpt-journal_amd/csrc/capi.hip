@@ -29,10 +29,8 @@ const rule_kernels& kernels_of_rule(int id) {
 namespace {
 
 constexpr int RULES = 10;
-thread_local std::string g_error;
-
 int fail(int code, const std::string& msg) {
-    g_error = msg;
+    last_error() = msg;
     return code;
 }
 
@@ -148,6 +146,6 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
                       static_cast<hipStream_t>(stream), result_in_b);
 }
 
-const char* cellato_hip_last_error(void) { return g_error.c_str(); }
+const char* cellato_hip_last_error(void) { return last_error().c_str(); }
 
 } // extern "C"

@@ -34,6 +34,7 @@ struct rule_kernels {
     int state_values[32];        // enum value of each dictionary index (for checksums)
     const void* simple[2][SIMPLE_VARIANTS];                   // [layout][variant]
     const void* temporal[2][TEMPORAL_VARIANTS][MAX_T + 1];    // [layout][variant][T], T >= 1
+    const void* batch[MAX_T + 1];                             // linear layout, [T], T >= 1
 };
 
 template <class Eval, int L>
@@ -65,6 +66,10 @@ rule_kernels make_rule_kernels() {
         k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
     fill_layout<Eval, LINEAR>(k);
     fill_layout<Eval, TILED>(k);
+    k.batch[0] = nullptr;
+    [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
+        ((k.batch[Ts + 1] = reinterpret_cast<const void*>(&batch_kernel<Eval, LINEAR, Ts + 1>)), ...);
+    }(std::make_integer_sequence<int, MAX_T>{});
     return k;
 }
 

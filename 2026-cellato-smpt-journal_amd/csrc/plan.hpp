@@ -21,6 +21,12 @@ inline std::string& plan_error() {
     return e;
 }
 
+// message of the last failed C-ABI call on this thread (cellato_hip_last_error)
+inline std::string& last_error() {
+    static thread_local std::string e;
+    return e;
+}
+
 // Default rows per task: long enough that the 2T rows of pipeline fill and drain stay a
 // small fraction of the rows streamed, short enough to give the chip a few thousand waves.
 inline int default_chunk(int rows_per_group_total, int T) {
@@ -47,24 +53,28 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         a.groups = 64 / wx;
         a.stride = 0;
         a.nseg = 1;
+        a.variant = VARIANT_GROUPS;
         pl.fn = temporal ? k.temporal[layout][TEMPORAL_GROUPS][T] : k.simple[layout][SIMPLE_GROUPS];
     } else if (wx == 64) {
         a.n = 64;
         a.groups = 1;
         a.stride = 0;
         a.nseg = 1;
+        a.variant = VARIANT_RING64;
         pl.fn = temporal ? k.temporal[layout][TEMPORAL_RING64][T] : k.simple[layout][SIMPLE_RING64];
     } else if (!temporal) {
         a.n = 64;
         a.groups = 1;
         a.stride = 64;
         a.nseg = (wx + 63) / 64;
+        a.variant = -1;   // halo loads: not used by the batch kernel
         pl.fn = k.simple[layout][SIMPLE_SEGMENT];
     } else {
         a.n = 64;
         a.groups = 1;
         a.stride = 62;
         a.nseg = (wx + 61) / 62;
+        a.variant = VARIANT_SEGMENT;
         pl.fn = k.temporal[layout][TEMPORAL_SEGMENT][T];
     }
     if (!pl.fn) {

@@ -57,7 +57,10 @@ struct pass_args {
     int nseg;        // segments per row
     int nchunk;      // row chunks
     int ntasks;      // nseg * nchunk
+    int variant;     // lane mapping, for the batch kernel: 0 segments + halo lanes, 1 ring of 64, 2 groups
 };
+
+enum lane_variant : int { VARIANT_SEGMENT = 0, VARIANT_RING64 = 1, VARIANT_GROUPS = 2 };
 
 __device__ __forceinline__ int lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -482,6 +485,37 @@ __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
     const int task = __builtin_amdgcn_readfirstlane(blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
     if (task >= a.ntasks) return;
     stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
+}
+
+// ---- batches: many grids, one persistent launch per (rule, T) and round ---------------------
+struct batch_task {
+    int pass;    // index into the pass table
+    int task;    // task of that pass (segment + chunk)
+};
+
+struct batch_args {
+    const pass_args* passes;
+    const batch_task* tasks;
+    int ntasks;
+};
+
+// A fixed grid of waves walks a task list that mixes passes of different grids (sizes, lane
+// mappings, start steps); each task runs exactly as in the single-grid kernel.
+template <class Eval, int LAYOUT, int T>
+__global__ void __launch_bounds__(BLOCK_THREADS) batch_kernel(batch_args b) {
+    const int nwaves = static_cast<int>(gridDim.x) * WAVES_PER_BLOCK;
+    const int first = __builtin_amdgcn_readfirstlane(xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)) *
+                                                     WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
+    for (int i = first; i < b.ntasks; i += nwaves) {
+        const batch_task t = b.tasks[i];
+        const pass_args& a = b.passes[t.pass];
+        if (a.variant == VARIANT_SEGMENT)
+            stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES>(a, t.task);
+        else if (a.variant == VARIANT_RING64)
+            stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE>(a, t.task);
+        else
+            stream_task<Eval, LAYOUT, T, XCH_PERMUTE, HALO_NONE>(a, t.task);
+    }
 }
 
 } // namespace cellato::hip

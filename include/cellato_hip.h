@@ -69,6 +69,33 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
                          int width, int height, int t0, int steps, int temporal_steps,
                          int chunk_rows, void* stream, int* result_in_b);
 
+/* ---- batches ------------------------------------------------------------------------------
+ * Many independent grids (any of the rules, any size, own step count, start step and depth)
+ * advanced together: the steps of every grid are cut into launches of its own depth, the
+ * launches of all grids are grouped by (rule, depth) into persistent work-list kernels, and
+ * the whole schedule is captured once into a hipGraph; cellato_batch_launch replays it with a
+ * single graph launch. Linear layout. The batch keeps pointers to the caller's buffers:
+ * they must stay allocated (and be re-filled with initial states before a replay). After a
+ * launch, grid i's result is in b if cellato_batch_result_in_b(handle, i) == 1, else in a.
+ * No counterpart in the reference (which advances one grid per traverser). */
+typedef struct cellato_batch_grid {
+    int rule;
+    int layout;             /* CELLATO_LAYOUT_LINEAR */
+    int width, height;      /* cells */
+    int steps;              /* generations to advance */
+    int t0;                 /* time step of the first generation */
+    int temporal_steps;     /* generations per launch, 1..cellato_max_temporal_steps() */
+    int chunk_rows;         /* 0 = default */
+    uint64_t* a[5];         /* P device plane pointers holding the initial state */
+    uint64_t* b[5];         /* P device plane pointers of the second buffer */
+} cellato_batch_grid;
+
+int cellato_batch_create(const cellato_batch_grid* grids, int n, int max_waves, void** handle);
+int cellato_batch_launch(void* handle, void* stream);
+int cellato_batch_result_in_b(void* handle, int grid);
+int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tasks);
+int cellato_batch_destroy(void* handle);
+
 /* Message of the last failed call on this host thread ("" if none). */
 const char* cellato_hip_last_error(void);
 
