@@ -71,6 +71,17 @@ def lib():
         L.cellato_batch_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
         L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.cellato_hip_multi.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.cellato_hip_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       _PP64, ctypes.c_void_p, ctypes.c_void_p]
+        L.cellato_hip_unpack.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+        L.cellato_hip_checksum.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_void_p]
+        L.cellato_hip_fill_random.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                              ctypes.c_void_p]
         _lib = L
     return _lib
 

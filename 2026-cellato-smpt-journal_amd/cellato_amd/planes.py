@@ -66,3 +66,47 @@ class DeviceGrid:
 
     def to_plane_grid(self):
         return PlaneGrid(self.rule, self.width, self.height, self.download(), self.layout)
+
+    # ---- on-device conversions (cellato_hip_pack / unpack / checksum / fill_random) ----
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+
+    def fill_random(self, seed, probs=None):
+        """Seeded i.i.d. initial state written on the device into the current buffer."""
+        import ctypes
+        from . import _lib
+        from .rules import iid_index_probs
+        probs = iid_index_probs(self.rule.name) if probs is None else probs
+        arr = (ctypes.c_double * len(probs))(*probs)
+        _lib.check(_lib.lib().cellato_hip_fill_random(self.rule.id, self.layout, _lib.plane_array(self.ptrs(self.cur)),
+                                                      self.width, self.height, seed, arr, len(probs), self._stream()))
+
+    def pack_values(self, values):
+        """values: torch int32 tensor (H, W) of state values on the device -> current buffer."""
+        import torch
+        from . import _lib
+        v = values.contiguous()
+        bad = torch.zeros(1, dtype=torch.int32, device=v.device)
+        _lib.check(_lib.lib().cellato_hip_pack(self.rule.id, self.layout, v.data_ptr(), self.width, self.height,
+                                               _lib.plane_array(self.ptrs(self.cur)), bad.data_ptr(), self._stream()))
+        if int(bad.item()):
+            raise ValueError(f"{self.rule.name}: value not in the state dictionary")
+
+    def unpack_values(self):
+        """Current buffer -> torch int32 tensor (H, W) of state values on the device."""
+        import torch
+        from . import _lib
+        out = torch.empty((self.height, self.width), dtype=torch.int32, device=self.buf.device)
+        _lib.check(_lib.lib().cellato_hip_unpack(self.rule.id, self.layout, _lib.plane_array(self.ptrs(self.cur)),
+                                                 self.width, self.height, out.data_ptr(), self._stream()))
+        return out
+
+    def checksum_device(self):
+        """The reference's 16-tile checksum, computed on the device."""
+        import torch
+        from . import _lib
+        out = torch.zeros(16, dtype=torch.int64, device=self.buf.device)
+        _lib.check(_lib.lib().cellato_hip_checksum(self.rule.id, self.layout, _lib.plane_array(self.ptrs(self.cur)),
+                                                   self.width, self.height, out.data_ptr(), self._stream()))
+        return [int(x) for x in out.cpu().tolist()]

@@ -86,3 +86,48 @@ def iid_values(name, width, height, seed):
     u = rng.random((height, width))
     idx = np.searchsorted(cum, u, side="right").clip(max=len(vals) - 1)
     return np.asarray(vals, np.int32)[idx]
+
+
+def iid_index_probs(name):
+    """The i.i.d. distribution of `iid_values` as probabilities per dictionary index (for the
+    device generator `DeviceGrid.fill_random`)."""
+    r = rule(name)
+    probs = [0.0] * len(r.states)
+    if name == "fluid":
+        for v in range(16):
+            k = bin(v).count("1")
+            probs[r.states.index(v)] = 0.25 ** k * 0.75 ** (4 - k)
+    elif name == "cyclic":
+        probs = [1.0 / 32] * 32
+    else:
+        for v, p in IID[name]:
+            probs[r.states.index(v)] += p
+    return probs
+
+
+def mix64(z):
+    """splitmix64 finaliser on numpy uint64 arrays (the device generator's hash)."""
+    z = np.asarray(z, np.uint64)
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def device_random_values(name, width, height, seed, probs=None):
+    """Host restatement of the device generator (cellato_hip_fill_random): the state values
+    it writes for this rule, size and seed."""
+    r = rule(name)
+    probs = iid_index_probs(name) if probs is None else probs
+    total = sum(probs)
+    cum, acc = [], 0.0
+    for p in probs:
+        acc += p / total
+        t = acc * 4294967296.0
+        cum.append(0xFFFFFFFF if t >= 4294967295.0 else int(t))
+    cum[-1] = 0xFFFFFFFF
+    cell = np.arange(width * height, dtype=np.uint64)
+    u = (mix64(np.uint64(seed) ^ mix64(cell)) >> np.uint64(32)).astype(np.uint64)
+    idx = np.searchsorted(np.asarray(cum[:-1], np.uint64), u, side="right")
+    return np.asarray(r.states, np.int32)[idx].reshape(height, width)

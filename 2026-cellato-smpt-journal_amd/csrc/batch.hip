@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <string>
@@ -61,11 +62,20 @@ struct batch {
         if (e_ != hipSuccess) return bfail(CELLATO_E_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
+// Host schedule of a set of grids: pass table, task lists, launches per round.
+struct schedule {
     std::vector<pass_args> passes;
-    // per round: (rule, T) -> list of (pass index, tasks of that pass)
+    std::vector<batch_task> tasks;
+    std::vector<std::vector<launch_rec>> rounds;
+    std::vector<int> result_in_b;
+    int launches = 0;
+};
+
+int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_waves) {
+    std::vector<pass_args>& passes = S.passes;
+    // per round: (rule, T) -> list of pass indices
     std::vector<std::map<std::pair<int, int>, std::vector<int>>> rounds;
-    B.result_in_b.assign(n, 0);
+    S.result_in_b.assign(n, 0);
     for (int gi = 0; gi < n; ++gi) {
         const cellato_batch_grid& g = grids[gi];
         if (g.rule < 0 || g.rule >= 10) return bfail(CELLATO_E_RULE, "grid " + std::to_string(gi) + ": unknown rule");
@@ -95,26 +105,36 @@ int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
             done += T;
             ++r;
         }
-        B.result_in_b[gi] = r % 2;
+        S.result_in_b[gi] = r % 2;
     }
-
-    std::vector<batch_task> tasks;
-    B.rounds.resize(rounds.size());
+    S.rounds.resize(rounds.size());
     for (size_t r = 0; r < rounds.size(); ++r) {
         for (auto& [key, plist] : rounds[r]) {
             launch_rec L;
             L.fn = kernels_of_rule(key.first).batch[key.second];
-            L.task_begin = static_cast<int>(tasks.size());
+            L.task_begin = static_cast<int>(S.tasks.size());
             for (int pi : plist)
-                for (int t = 0; t < passes[pi].ntasks; ++t) tasks.push_back({pi, t});
-            L.task_count = static_cast<int>(tasks.size()) - L.task_begin;
+                for (int t = 0; t < passes[pi].ntasks; ++t) S.tasks.push_back({pi, t});
+            L.task_count = static_cast<int>(S.tasks.size()) - L.task_begin;
             const int waves = std::min(L.task_count, max_waves);
             L.blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-            B.rounds[r].push_back(L);
-            ++B.launches;
+            S.rounds[r].push_back(L);
+            ++S.launches;
         }
     }
-    B.tasks = static_cast<long long>(tasks.size());
+    return CELLATO_OK;
+}
+
+int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
+    schedule S;
+    const int prc = plan_schedule(S, grids, n, max_waves);
+    if (prc != CELLATO_OK) return prc;
+    B.result_in_b = S.result_in_b;
+    B.rounds = S.rounds;
+    B.launches = S.launches;
+    B.tasks = static_cast<long long>(S.tasks.size());
+    const std::vector<pass_args>& passes = S.passes;
+    const std::vector<batch_task>& tasks = S.tasks;
 
     const size_t pbytes = passes.size() * sizeof(pass_args);
     const size_t tbytes = tasks.size() * sizeof(batch_task);
@@ -160,6 +180,59 @@ int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
     return CELLATO_OK;
 }
 
+// Tables of cellato_hip_multi: two device regions and two pinned host staging buffers used
+// alternately; a region is rewritten only after the kernels that read it have finished.
+struct multi_state {
+    void* dev[2] = {nullptr, nullptr};
+    void* host[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int next = 0;
+};
+
+multi_state& mstate() {
+    static thread_local multi_state m;
+    return m;
+}
+
+int run_multi(const cellato_batch_grid* grids, int n, int max_waves, hipStream_t stream, int* result_in_b) {
+    schedule S;
+    const int prc = plan_schedule(S, grids, n, max_waves);
+    if (prc != CELLATO_OK) return prc;
+    for (int i = 0; i < n; ++i) result_in_b[i] = S.result_in_b[i];
+    if (S.tasks.empty()) return CELLATO_OK;
+    multi_state& M = mstate();
+    const int k = M.next;
+    M.next ^= 1;
+    const size_t pbytes = ((S.passes.size() * sizeof(pass_args) + 255) / 256) * 256;
+    const size_t bytes = pbytes + S.tasks.size() * sizeof(batch_task);
+    if (!M.done[k]) BCHECK(hipEventCreateWithFlags(&M.done[k], hipEventDisableTiming));
+    BCHECK(hipEventSynchronize(M.done[k]));   // kernels of the call before last are finished
+    if (bytes > M.cap[k]) {
+        if (M.dev[k]) BCHECK(hipFree(M.dev[k]));
+        if (M.host[k]) BCHECK(hipHostFree(M.host[k]));
+        M.dev[k] = M.host[k] = nullptr;
+        M.cap[k] = 0;
+        const size_t want = bytes * 2;
+        BCHECK(hipMalloc(&M.dev[k], want));
+        BCHECK(hipHostMalloc(&M.host[k], want, hipHostMallocDefault));
+        M.cap[k] = want;
+    }
+    std::memcpy(M.host[k], S.passes.data(), S.passes.size() * sizeof(pass_args));
+    std::memcpy(static_cast<char*>(M.host[k]) + pbytes, S.tasks.data(), S.tasks.size() * sizeof(batch_task));
+    BCHECK(hipMemcpyAsync(M.dev[k], M.host[k], bytes, hipMemcpyHostToDevice, stream));
+    const auto* dpass = static_cast<const pass_args*>(M.dev[k]);
+    const auto* dtask = reinterpret_cast<const batch_task*>(static_cast<char*>(M.dev[k]) + pbytes);
+    for (auto& round : S.rounds)
+        for (const launch_rec& L : round) {
+            batch_args args{dpass, dtask + L.task_begin, L.task_count};
+            void* kargs[] = {&args};
+            BCHECK(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, stream));
+        }
+    BCHECK(hipEventRecord(M.done[k], stream));
+    return CELLATO_OK;
+}
+
 } // namespace
 } // namespace cellato::hip
 
@@ -199,6 +272,11 @@ int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tas
     if (launches) *launches = B->launches;
     if (tasks) *tasks = B->tasks;
     return CELLATO_OK;
+}
+
+int cellato_hip_multi(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b) {
+    if ((n > 0 && (!grids || !result_in_b)) || n < 0) return bfail(CELLATO_E_ARG, "null argument");
+    return run_multi(grids, n, max_waves > 0 ? max_waves : 8192, static_cast<hipStream_t>(stream), result_in_b);
 }
 
 int cellato_batch_destroy(void* handle) {

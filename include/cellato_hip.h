@@ -96,6 +96,32 @@ int cellato_batch_result_in_b(void* handle, int grid);
 int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tasks);
 int cellato_batch_destroy(void* handle);
 
+/* The same schedule launched directly (no graph) on `stream`, for callers whose set of grids
+ * changes between calls (the serving loop calls it once per tick with one pass per live job).
+ * result_in_b[i] receives which buffer holds grid i's result. The pass tables go through two
+ * pinned staging buffers used alternately (kept for the life of the thread). */
+int cellato_hip_multi(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b);
+
+/* ---- conversions on the device -------------------------------------------------------------
+ * values: device array of width*height int32 state values (enum values, row-major), the
+ * reference's standard grid. Asynchronous on `stream`.
+ * pack:     values -> planes (bit_planes_grid.hpp:234-270, tiled :243-288); *bad_values (a
+ *           device int, zeroed by the caller) becomes nonzero if a value is not a state.
+ * unpack:   planes -> values (get_cell, bit_planes_grid.hpp:97-124).
+ * checksum: the 16 tile sums of state values of standard_grid.hpp:179-213, added into the
+ *           device array sums16 (zero it first).
+ * fill_random: seeded i.i.d. state per cell: dictionary index k with probability probs[k]
+ *           (host array, one per index, normalised); cell (x, y) draws the top 32 bits of
+ *           splitmix64(seed ^ splitmix64(y * width + x)). */
+int cellato_hip_pack(int rule, int layout, const int32_t* values, int width, int height, uint64_t* const* planes,
+                     int* bad_values, void* stream);
+int cellato_hip_unpack(int rule, int layout, uint64_t* const* planes, int width, int height, int32_t* values,
+                       void* stream);
+int cellato_hip_checksum(int rule, int layout, uint64_t* const* planes, int width, int height,
+                         unsigned long long* sums16, void* stream);
+int cellato_hip_fill_random(int rule, int layout, uint64_t* const* planes, int width, int height,
+                            unsigned long long seed, const double* probs, int nprobs, void* stream);
+
 /* Message of the last failed call on this host thread ("" if none). */
 const char* cellato_hip_last_error(void);
 
