@@ -57,6 +57,7 @@ def lib():
         L.cellato_rule_planes.argtypes = [ctypes.c_int]
         L.cellato_rule_states.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
         L.cellato_max_temporal_steps.restype = ctypes.c_int
+        L.cellato_rule_traits.argtypes = [ctypes.c_int]
         L.cellato_hip_last_error.restype = ctypes.c_char_p
         L.cellato_hip_simple.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, _PP64, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -82,6 +83,14 @@ def lib():
         L.cellato_hip_fill_random.argtypes = [ctypes.c_int, ctypes.c_int, _PP64, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                               ctypes.c_void_p]
+        vp = ctypes.c_void_p
+        L.cellato_mlp_linear_forward.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, vp]
+        L.cellato_mlp_linear_backward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_int, vp]
+        L.cellato_mlp_mse.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+        L.cellato_mlp_adam.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_int, vp]
         _lib = L
     return _lib
 

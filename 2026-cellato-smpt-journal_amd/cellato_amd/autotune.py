@@ -1,10 +1,12 @@
-"""Autotuner: the temporal depth T and the rows per task (`chunk_rows`) per (automaton, size).
+"""Autotuner: the temporal depth T per (automaton, size).
 
-A configuration is measured as throughput, the way a batch uses it: enough copies of a grid
-of the bucket's size to fill the GPU, advanced by one graph launch (`Batch`), timed with
-HIP events. Sizes are bucketed by powers of two of width and height. Without a cost model
-every candidate of a bucket is measured; with one (`cellato_amd.costmodel.CostModel`) only the
-`budget` candidates it ranks best are measured. Results are cached in a JSON file.
+The rows per task are not tuned here: the launch planner picks them per launch round from
+the round's total work (csrc/plan.hpp, auto_chunk). A depth is measured as throughput, the
+way a batch uses it: enough copies of a grid of the bucket's size to fill the GPU, advanced
+by one graph launch (`Batch`), timed with HIP events over a few replays. Sizes are bucketed
+by powers of two of width and height. Without a cost model every depth of a bucket is
+measured; with one (`cellato_amd.costmodel.CostModel`) only the `budget` depths it ranks
+best are measured. Results are cached in a JSON file.
 """
 import json
 import math
@@ -15,34 +17,35 @@ import numpy as np
 from .rules import rule as get_rule
 
 DEPTHS = (1, 2, 3, 4, 5, 6, 8)
-CHUNKS = (0, 8, 32, 64)
-CANDIDATES = [(T, c) for T in DEPTHS for c in CHUNKS]
+CANDIDATES = [(T, 0) for T in DEPTHS]
 
 
 def bucket(side):
     return int(2 ** max(6, min(15, round(math.log2(max(side, 1))))))
 
 
-def features(rule_name, width, height, T, chunk):
-    """Inputs of the cost model: rule one-hot, planes, log sizes, depth, log chunk, lane mapping."""
+def features(rule_name, width, height, T, chunk=0):
+    """Inputs of the cost model: what the rule is made of (planes and the kinds of nodes of
+    its AST, not its identity, so that the model says something about rules it has not
+    seen), the grid size, the depth, and the lane mapping the size implies."""
     r = get_rule(rule_name)
-    onehot = [0.0] * 10
-    onehot[r.id] = 1.0
     wx = width // 64
-    eff_chunk = chunk if chunk > 0 else max(16, 4 * T)
-    return onehot + [r.planes / 5.0, math.log2(width) / 15.0, math.log2(height) / 15.0, T / 8.0,
-                     math.log2(eff_chunk) / 8.0, 1.0 if wx < 64 else 0.0, 1.0 if wx == 64 else 0.0,
-                     (2.0 * T / eff_chunk), 1.0 / T]
+    return [r.planes / 5.0, math.log2(width) / 15.0, math.log2(height) / 15.0, T / 8.0, 1.0 / T,
+            (T / 8.0) ** 2, r.planes * T / 40.0,
+            float(r.has("moore")), float(r.has("von_neumann")), float(r.has("margolus")),
+            float(r.has("neighbor")), float(r.has("dynamic_target")), float(r.has("alternating")),
+            1.0 if wx < 64 else 0.0, 1.0 if wx == 64 else 0.0]
 
 
 class Tuner:
-    def __init__(self, cache_path=None, model=None, budget=4, target_cells=1 << 26, steps=24, measure=True):
+    def __init__(self, cache_path=None, model=None, budget=3, target_cells=1 << 27, steps=24, replays=2, measure=True):
         self.cache_path = Path(cache_path) if cache_path else None
         self.cache = {}
         if self.cache_path and self.cache_path.exists():
             self.cache = json.loads(self.cache_path.read_text())
         self.model, self.budget = model, budget
         self.target_cells, self.steps, self.measure_enabled = target_cells, steps, measure
+        self.replays = replays
         self.measurements = 0
 
     @staticmethod
@@ -73,25 +76,26 @@ class Tuner:
         return [CANDIDATES[i] for i in order[: self.budget]]
 
     def measure(self, rule_name, width, height, T, chunk):
-        """GCUPS of `copies` grids of this size advanced `steps` generations in one batch."""
+        """GCUPS of `copies` grids of this size advanced `steps` generations in one batch,
+        timed over `replays` graph launches (the grids keep evolving; only time is read)."""
         import torch
         from .batch import Batch, Job
-        copies = max(1, min(512, self.target_cells // (width * height)))
-        jobs = [Job(rule_name, width, height, self.steps, temporal_steps=T, chunk_rows=chunk, seed=i)
+        copies = max(1, min(2048, self.target_cells // (width * height)))
+        jobs = [Job(rule_name, width, height, self.steps, temporal_steps=T, chunk_rows=chunk, seed=i, on_device=True)
                 for i in range(copies)]
         b = Batch(jobs)
         b.run()                      # warm-up replay
         torch.cuda.synchronize()
-        b.reset()
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record()
-        b.run()
+        for _ in range(self.replays):
+            b.run()
         stop.record()
         torch.cuda.synchronize()
         ms = start.elapsed_time(stop)
         b.close()
         self.measurements += 1
-        return copies * width * height * self.steps / (ms * 1e-3) / 1e9
+        return copies * width * height * self.steps * self.replays / (ms * 1e-3) / 1e9
 
     def tune(self, rule_name, width, height):
         results = []
@@ -99,3 +103,32 @@ class Tuner:
             results.append({"T": T, "chunk": chunk, "gcups": self.measure(rule_name, width, height, T, chunk)})
         best = max(results, key=lambda r: r["gcups"])
         return {"T": best["T"], "chunk": best["chunk"], "gcups": best["gcups"], "measured": results}
+
+
+SIDES = (64, 128, 256, 512, 1024, 2048, 4096, 8192)
+
+
+def collect_dataset(path, rules=None, sides=SIDES, log=print):
+    """Measure every candidate of every (rule, square side) bucket and write the samples
+    (features and measured ns per cell update) to `path` as JSON."""
+    from .rules import all_rules
+    names = rules or [r.name for r in all_rules()]
+    tuner = Tuner(measure=True)
+    rows = []
+    for name in names:
+        for side in sides:
+            for T, chunk in CANDIDATES:
+                g = tuner.measure(name, side, side, T, chunk)
+                rows.append({"rule": name, "width": side, "height": side, "T": T, "chunk": chunk, "gcups": g,
+                             "ns_per_cell": 1.0 / g})
+            best = max((r for r in rows if r["rule"] == name and r["width"] == side), key=lambda r: r["gcups"])
+            log(f"{name} {side}: best T={best['T']} chunk={best['chunk']} {best['gcups']:.0f} GCUPS")
+    Path(path).write_text(json.dumps({"device": "MI355X", "steps": tuner.steps, "replays": tuner.replays,
+                                      "target_cells": tuner.target_cells, "samples": rows}))
+    return rows
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) >= 3 and sys.argv[1] == "collect":
+        collect_dataset(sys.argv[2], rules=sys.argv[3].split(",") if len(sys.argv) > 3 else None)

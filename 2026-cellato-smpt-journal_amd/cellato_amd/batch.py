@@ -26,7 +26,8 @@ class Job:
     temporal_steps: int = 0      # 0: ask the tuner
     chunk_rows: int = 0
     seed: int = 0
-    values: np.ndarray = field(default=None, repr=False)   # initial state values; default: seeded i.i.d.
+    values: np.ndarray = field(default=None, repr=False)   # initial state values (host), or
+    on_device: bool = False      # True: seeded i.i.d. state generated on the device (fill_random)
 
     @property
     def cells(self):
@@ -52,8 +53,15 @@ def batch_workload(n=512, seed=2026, rules=BATCH_RULES, side_range=(64, 4096), s
         side = int(round(math.exp(rng.uniform(lo, hi)) / 64)) * 64
         side = min(max(side, side_range[0]), side_range[1])
         steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
-        jobs.append(Job(name, side, side, steps, seed=seed * 100003 + i))
+        jobs.append(Job(name, side, side, steps, seed=seed * 100003 + i, on_device=True))
     return jobs
+
+
+def _init(d, j):
+    if j.on_device:
+        d.fill_random(j.seed)
+    else:
+        d.upload(hg.pack(d.rule.index_of(j.initial_values()), d.rule.planes, hg.LINEAR))
 
 
 def _stream():
@@ -72,7 +80,7 @@ class Batch:
             if T <= 0:
                 T, chunk = tuner.pick(j.rule, j.width, j.height) if tuner else (4, 0)
             d = DeviceGrid(r, j.width, j.height, hg.LINEAR)
-            d.upload(hg.pack(r.index_of(j.initial_values()), r.planes, hg.LINEAR))
+            _init(d, j)
             self.grids.append(d)
             g = arr[i]
             g.rule, g.layout, g.width, g.height = r.id, hg.LINEAR, j.width, j.height
@@ -98,17 +106,25 @@ class Batch:
         """Put every grid's initial state back into its first buffer (before a replay)."""
         for j, d in zip(self.jobs, self.grids):
             d.cur = 0
-            d.upload(hg.pack(d.rule.index_of(j.initial_values()), d.rule.planes, hg.LINEAR))
+            _init(d, j)
 
     def run(self, stream=None):
         _lib.check(_lib.lib().cellato_batch_launch(self.handle, ctypes.c_void_p(stream if stream is not None else _stream())))
 
+    def _settle(self, i):
+        d = self.grids[i]
+        d.cur = _lib.check(_lib.lib().cellato_batch_result_in_b(self.handle, i))
+        return d
+
     def result(self, i):
         import torch
         torch.cuda.synchronize()
-        d = self.grids[i]
-        d.cur = _lib.check(_lib.lib().cellato_batch_result_in_b(self.handle, i))
+        d = self._settle(i)
         return d.rule.value_of(hg.unpack(d.download(), d.width, d.height, hg.LINEAR))
+
+    def checksum(self, i):
+        """16-tile checksum of grid i's result, computed on the device."""
+        return self._settle(i).checksum_device()
 
     def close(self):
         if self.handle:
@@ -134,13 +150,13 @@ class PlainLoop:
         for j in self.jobs:
             r = get_rule(j.rule)
             d = DeviceGrid(r, j.width, j.height, hg.LINEAR)
-            d.upload(hg.pack(r.index_of(j.initial_values()), r.planes, hg.LINEAR))
+            _init(d, j)
             self.grids.append(d)
 
     def reset(self):
         for j, d in zip(self.jobs, self.grids):
             d.cur = 0
-            d.upload(hg.pack(d.rule.index_of(j.initial_values()), d.rule.planes, hg.LINEAR))
+            _init(d, j)
 
     def run(self, temporal_steps=0, tuner=None):
         st = _stream()
@@ -157,3 +173,6 @@ class PlainLoop:
         torch.cuda.synchronize()
         d = self.grids[i]
         return d.rule.value_of(hg.unpack(d.download(), d.width, d.height, hg.LINEAR))
+
+    def checksum(self, i):
+        return self.grids[i].checksum_device()

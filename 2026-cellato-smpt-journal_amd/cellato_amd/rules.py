@@ -11,12 +11,20 @@ import numpy as np
 from . import _lib
 
 
+TRAITS = {"moore": 1, "von_neumann": 2, "margolus": 4, "neighbor": 8, "dynamic_target": 16,
+          "arithmetic": 32, "alternating": 64}
+
+
 @dataclass(frozen=True)
 class Rule:
     id: int
     name: str
     planes: int
     states: tuple  # enum value of each dictionary index
+    traits: int = 0  # rule_trait flags derived from the rule's AST at compile time
+
+    def has(self, trait):
+        return bool(self.traits & TRAITS[trait])
 
     def index_of(self, values):
         """Dictionary indices of an array of state values (enum values)."""
@@ -37,7 +45,8 @@ def _load():
     for r in range(L.cellato_rule_count()):
         buf = (ctypes.c_int32 * 32)()
         n = L.cellato_rule_states(r, buf, 32)
-        out.append(Rule(r, L.cellato_rule_name(r).decode(), L.cellato_rule_planes(r), tuple(buf[:n])))
+        out.append(Rule(r, L.cellato_rule_name(r).decode(), L.cellato_rule_planes(r), tuple(buf[:n]),
+                        L.cellato_rule_traits(r)))
     return out
 
 

@@ -75,6 +75,9 @@ int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_w
     std::vector<pass_args>& passes = S.passes;
     // per round: (rule, T) -> list of pass indices
     std::vector<std::map<std::pair<int, int>, std::vector<int>>> rounds;
+    struct pending_pass { int grid, T, done; };
+    std::vector<std::vector<pending_pass>> plan_T;   // per round: passes to plan
+    std::vector<long long> round_streams;           // per round: total row-streams of work
     S.result_in_b.assign(n, 0);
     for (int gi = 0; gi < n; ++gi) {
         const cellato_batch_grid& g = grids[gi];
@@ -91,21 +94,31 @@ int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_w
         int done = 0, r = 0;
         while (done < g.steps) {
             const int T = std::min(g.temporal_steps, g.steps - done);
+            if (static_cast<int>(plan_T.size()) <= r) { plan_T.resize(r + 1); round_streams.resize(r + 1, 0); }
+            plan_T[r].push_back({gi, T, done});
+            round_streams[r] += pass_streams(g.rule, g.layout, g.width, g.height, T, true);
+            done += T;
+            ++r;
+        }
+        S.result_in_b[gi] = r % 2;
+    }
+    // second pass: plan every pass with the chunk that fills the GPU for its round's total work
+    for (size_t r = 0; r < plan_T.size(); ++r) {
+        for (const auto& e : plan_T[r]) {
+            const cellato_batch_grid& g = grids[e.grid];
+            const int P = kernels_of_rule(g.rule).planes;
             pass_plan pl;
-            const int rc = plan_pass(g.rule, g.layout, g.width, g.height, T, true, g.chunk_rows, pl);
+            const int rc = plan_pass(g.rule, g.layout, g.width, g.height, e.T, true, g.chunk_rows, pl, round_streams[r]);
             if (rc != CELLATO_OK) return bfail(rc, plan_error());
             for (int p = 0; p < P; ++p) {
                 pl.args.in[p] = r % 2 == 0 ? g.a[p] : g.b[p];
                 pl.args.out[p] = r % 2 == 0 ? g.b[p] : g.a[p];
             }
-            pl.args.t0 = g.t0 + done;
-            if (static_cast<int>(rounds.size()) <= r) rounds.resize(r + 1);
-            rounds[r][{g.rule, T}].push_back(static_cast<int>(passes.size()));
+            pl.args.t0 = g.t0 + e.done;
+            if (rounds.size() <= r) rounds.resize(r + 1);
+            rounds[r][{g.rule, e.T}].push_back(static_cast<int>(passes.size()));
             passes.push_back(pl.args);
-            done += T;
-            ++r;
         }
-        S.result_in_b[gi] = r % 2;
     }
     S.rounds.resize(rounds.size());
     for (size_t r = 0; r < rounds.size(); ++r) {

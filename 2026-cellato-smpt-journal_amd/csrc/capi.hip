@@ -125,6 +125,11 @@ int cellato_rule_states(int rule, int32_t* values, int capacity) {
     return k.nstates;
 }
 
+int cellato_rule_traits(int rule) {
+    if (rule < 0 || rule >= RULES) return CELLATO_E_RULE;
+    return static_cast<int>(kernels_of_rule(rule).traits);
+}
+
 int cellato_max_temporal_steps(void) { return MAX_T; }
 
 int cellato_hip_simple(int rule, int layout, uint64_t* const* buf_a, uint64_t* const* buf_b, int width, int height,

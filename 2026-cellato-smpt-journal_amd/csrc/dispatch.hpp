@@ -31,6 +31,7 @@ struct rule_kernels {
     const char* name;
     int planes;
     int nstates;                 // states of the dictionary
+    unsigned traits;             // rule_trait flags of the rule's AST
     int state_values[32];        // enum value of each dictionary index (for checksums)
     const void* simple[2][SIMPLE_VARIANTS];                   // [layout][variant]
     const void* temporal[2][TEMPORAL_VARIANTS][MAX_T + 1];    // [layout][variant][T], T >= 1
@@ -62,6 +63,7 @@ rule_kernels make_rule_kernels() {
     k.name = Config::name;
     k.planes = Config::state_dictionary::needed_bits;
     k.nstates = Config::state_dictionary::number_of_values;
+    k.traits = cellato::evaluators::bit_planes::rule_traits<typename Config::algorithm>::value;
     for (int i = 0; i < k.nstates && i < 32; ++i)
         k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
     fill_layout<Eval, LINEAR>(k);

@@ -27,17 +27,25 @@ inline std::string& last_error() {
     return e;
 }
 
-// Default rows per task: long enough that the 2T rows of pipeline fill and drain stay a
-// small fraction of the rows streamed, short enough to give the chip a few thousand waves.
-inline int default_chunk(int rows_per_group_total, int T) {
-    int c = 16 > 4 * T ? 16 : 4 * T;
-    // small grids: do not make a chunk longer than the grid itself
-    if (c > rows_per_group_total) c = rows_per_group_total > 0 ? rows_per_group_total : 1;
-    return c;
+// Rows per task when the caller passes 0. A task streams `chunk` rows plus 2T rows of
+// pipeline fill and drain, so long chunks waste less; but a launch needs a few thousand
+// waves to keep 1024 SIMDs busy. Pick the longest chunk that still yields about
+// TARGET_TASKS tasks for `streams` row-streams of work (segments x rows over all grids of
+// the launch), never shorter than max(8, 2T) and never longer than the grid.
+constexpr int TARGET_TASKS = 3072;
+
+inline int auto_chunk(long long streams, int T, int rows_per_group) {
+    long long c = (streams + TARGET_TASKS - 1) / TARGET_TASKS;
+    const int lo = 2 * T > 8 ? 2 * T : 8;
+    if (c < lo) c = lo;
+    if (c > 256) c = 256;
+    if (c > rows_per_group) c = rows_per_group > 0 ? rows_per_group : 1;
+    return static_cast<int>(c);
 }
 
 // width / height in cells. T = generations per launch (1 for the simple traverser).
-inline int plan_pass(int rule, int layout, int width, int height, int T, bool temporal, int chunk, pass_plan& pl) {
+inline int plan_pass(int rule, int layout, int width, int height, int T, bool temporal, int chunk, pass_plan& pl,
+                     long long launch_streams = 0) {
     const rule_kernels& k = kernels_of_rule(rule);
     const int wx = layout == CELLATO_LAYOUT_LINEAR ? width / 64 : width / 8;
     const int h = layout == CELLATO_LAYOUT_LINEAR ? height : height / 8;
@@ -82,7 +90,8 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         return CELLATO_E_TEMPORAL;
     }
     const int rows_per_group = (h + a.groups - 1) / a.groups;
-    a.chunk = chunk > 0 ? chunk : default_chunk(rows_per_group, T);
+    const long long streams = launch_streams > 0 ? launch_streams : static_cast<long long>(a.nseg) * rows_per_group;
+    a.chunk = chunk > 0 ? chunk : auto_chunk(streams, T, rows_per_group);
     a.nchunk = (h + a.groups * a.chunk - 1) / (a.groups * a.chunk);
     const long long tasks = static_cast<long long>(a.nseg) * a.nchunk;
     if (tasks > (1LL << 30)) {
@@ -95,6 +104,14 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
     return CELLATO_OK;
 }
 
+// segments x rows per lane group of one pass: the unit of work auto_chunk divides
+inline long long pass_streams(int rule, int layout, int width, int height, int T, bool temporal) {
+    pass_plan pl;
+    if (plan_pass(rule, layout, width, height, T, temporal, 1, pl) != CELLATO_OK) return 0;
+    return static_cast<long long>(pl.args.nseg) * ((pl.args.h + pl.args.groups - 1) / pl.args.groups);
+}
+
 } // namespace cellato::hip
 
 #endif
+

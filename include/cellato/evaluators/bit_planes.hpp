@@ -484,6 +484,62 @@ struct ev<Dict, margolus_180_neighbor> {
     }
 };
 
+// ---- rule traits: which kinds of nodes a rule uses (compile-time walk of the AST) -----------
+enum rule_trait : unsigned {
+    TRAIT_MOORE = 1u,          // count over the Moore neighbourhood
+    TRAIT_VON_NEUMANN = 2u,    // count over the von Neumann neighbourhood
+    TRAIT_MARGOLUS = 4u,       // Margolus block count or 180-degree neighbour
+    TRAIT_NEIGHBOR = 8u,       // a single neighbour read directly
+    TRAIT_DYNAMIC_TARGET = 16u,// a count whose target state is computed per cell
+    TRAIT_ARITHMETIC = 32u,    // plus / modulo on states
+    TRAIT_ALTERNATING = 64u,   // alternate_algorithms (parity of the time step)
+};
+
+template <class E>
+struct rule_traits {
+    static constexpr unsigned value = 0;
+};
+template <template <class...> class N, class... A>
+struct rule_traits<N<A...>> {
+    static constexpr unsigned value = (0u | ... | rule_traits<A>::value);
+};
+template <int dx, int dy>
+struct rule_traits<neighbor_at<dx, dy>> {
+    static constexpr unsigned value = (dx == 0 && dy == 0) ? 0u : TRAIT_NEIGHBOR;
+};
+template <class V, int b>
+struct rule_traits<has_bit_set<V, b>> {
+    static constexpr unsigned value = rule_traits<V>::value;
+};
+template <class C>
+struct rule_traits<count_neighbors<C, moore_8_neighbors>> {
+    static constexpr unsigned value = TRAIT_MOORE | (is_state_literal<C>::value ? 0u : TRAIT_DYNAMIC_TARGET) | rule_traits<C>::value;
+};
+template <class C>
+struct rule_traits<count_neighbors<C, von_neumann_4_neighbors>> {
+    static constexpr unsigned value = TRAIT_VON_NEUMANN | (is_state_literal<C>::value ? 0u : TRAIT_DYNAMIC_TARGET) | rule_traits<C>::value;
+};
+template <class C>
+struct rule_traits<count_neighbors<C, margolus_alternating_neighborhood>> {
+    static constexpr unsigned value = TRAIT_MARGOLUS;
+};
+template <>
+struct rule_traits<margolus_180_neighbor> {
+    static constexpr unsigned value = TRAIT_MARGOLUS;
+};
+template <class L, class R>
+struct rule_traits<plus<L, R>> {
+    static constexpr unsigned value = TRAIT_ARITHMETIC | rule_traits<L>::value | rule_traits<R>::value;
+};
+template <class L, class R>
+struct rule_traits<modulo<L, R>> {
+    static constexpr unsigned value = TRAIT_ARITHMETIC | rule_traits<L>::value | rule_traits<R>::value;
+};
+template <class... A>
+struct rule_traits<alternate_algorithms<A...>> {
+    static constexpr unsigned value = TRAIT_ALTERNATING | (0u | ... | rule_traits<A>::value);
+};
+
 // The reference's evaluator name: evaluator<word, dictionary, Expression>
 // (evaluators/bit_planes.hpp:48-49). Only 64-bit words are implemented on the GPU.
 template <typename cell_row_type, typename state_dictionary_type, typename Expression>

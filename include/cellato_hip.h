@@ -46,6 +46,10 @@ int cellato_rule_planes(int rule);                /* bit planes P, or CELLATO_E_
  * `capacity` of them) and returns n; plane b of a cell is bit b of its dictionary index
  * (reference: include/memory/state_dictionary.hpp:10-70). */
 int cellato_rule_states(int rule, int32_t* values, int capacity);
+/* Kinds of nodes the rule's AST uses, as flags: 1 Moore count, 2 von Neumann count,
+ * 4 Margolus block, 8 direct neighbour read, 16 count with a per-cell target, 32 plus/modulo,
+ * 64 alternating rules (include/cellato/evaluators/bit_planes.hpp, rule_traits). */
+int cellato_rule_traits(int rule);
 int cellato_max_temporal_steps(void);
 
 /* ---- traversers --------------------------------------------------------------------------
@@ -121,6 +125,20 @@ int cellato_hip_checksum(int rule, int layout, uint64_t* const* planes, int widt
                          unsigned long long* sums16, void* stream);
 int cellato_hip_fill_random(int rule, int layout, uint64_t* const* planes, int width, int height,
                             unsigned long long seed, const double* probs, int nprobs, void* stream);
+
+/* ---- cost-model network (autotuner) -------------------------------------------------------
+ * fp32, row-major, device pointers, asynchronous on `stream`. No counterpart in the reference.
+ * linear_forward:  Y[N, Dout] = act(X[N, Din] W[Din, Dout] + b[Dout]); act = ReLU if relu.
+ * linear_backward: G = dY * act'(Y); dW = X^T G; db = column sums of G (if db); dX = G W^T (if dX).
+ * mse:             dpred = 2 (pred - target) / N; *loss += mean squared error (zero it first).
+ * adam:            one Adam step (bias-corrected, step >= 1) on n parameters. */
+int cellato_mlp_linear_forward(const float* X, const float* W, const float* b, float* Y, int N, int Din, int Dout,
+                               int relu, void* stream);
+int cellato_mlp_linear_backward(const float* X, const float* W, const float* Y, const float* dY, float* G, float* dX,
+                                float* dW, float* db, int N, int Din, int Dout, int relu, void* stream);
+int cellato_mlp_mse(const float* pred, const float* target, float* dpred, float* loss, int N, void* stream);
+int cellato_mlp_adam(float* param, const float* grad, float* m, float* v, int n, float lr, float beta1, float beta2,
+                     float eps, int step, void* stream);
 
 /* Message of the last failed call on this host thread ("" if none). */
 const char* cellato_hip_last_error(void);
