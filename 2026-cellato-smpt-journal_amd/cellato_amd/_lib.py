@@ -72,6 +72,9 @@ def lib():
         L.cellato_batch_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
         L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.cellato_batch_plan.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
         L.cellato_hip_multi.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_int)]
         L.cellato_hip_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,

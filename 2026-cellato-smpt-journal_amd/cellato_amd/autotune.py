@@ -17,6 +17,7 @@ import numpy as np
 from .rules import rule as get_rule
 
 DEPTHS = (1, 2, 3, 4, 5, 6, 8)
+BATCH_DEPTHS = (2, 4, 6, 8)
 CANDIDATES = [(T, 0) for T in DEPTHS]
 
 
@@ -96,6 +97,75 @@ class Tuner:
         b.close()
         self.measurements += 1
         return copies * width * height * self.steps * self.replays / (ms * 1e-3) / 1e9
+
+    def ns_per_cell(self, rule_name, width, height, T):
+        """Time per cell update at depth T for this bucket: measured when the bucket's tuning
+        measured T, else the cost model's prediction (tuning the bucket first if needed)."""
+        self.pick(rule_name, width, height)
+        e = self.cache.get(self.key(rule_name, width, height))
+        for m in (e or {}).get("measured", []):
+            if m["T"] == T and m["gcups"] > 0:
+                return 1.0 / m["gcups"]
+        if self.model is not None:
+            x = np.array([features(rule_name, bucket(width), bucket(height), T)], np.float32)
+            return float(np.exp(self.model.predict(x)[0]))
+        return float("inf")
+
+    def automaton_depths(self, jobs):
+        """One depth per automaton: the depth with the least predicted time over that
+        automaton's jobs (cell updates x time per cell update of each job's bucket)."""
+        by_rule = {}
+        for j in jobs:
+            by_rule.setdefault(j.rule, []).append(j)
+        best = {}
+        for name, js in by_rule.items():
+            cost = {T: sum(j.cells * j.steps * self.ns_per_cell(name, j.width, j.height, T) for j in js)
+                    for T in DEPTHS}
+            best[name] = min(DEPTHS, key=lambda T: (cost[T], -T))
+        return [best[j.rule] for j in jobs]
+
+    def batch_candidates(self, jobs):
+        """Depth assignments worth timing on a batch: one depth for all grids (BATCH_DEPTHS),
+        one per automaton, and the per-(automaton, size) picks."""
+        cands = {f"T{T}": [T] * len(jobs) for T in BATCH_DEPTHS}
+        cands["per_automaton"] = self.automaton_depths(jobs)
+        cands["per_automaton_and_size"] = [self.pick(j.rule, j.width, j.height)[0] for j in jobs]
+        return cands
+
+    def batch_depths(self, jobs):
+        """Depth per job of a batch, measured on the batch itself: every candidate of
+        `batch_candidates` is built (`Batch`) and replayed once after a warm-up replay, and
+        the fastest is kept. A batch's time is set by its rounds (one launch per automaton and
+        depth in each round, each paying a pass's pipeline fill), which per-bucket
+        measurements do not see, so the batch is timed whole. Returns (depths, name of the
+        winner, {candidate: GCUPS})."""
+        import torch
+        from .batch import Batch
+        saved = [j.temporal_steps for j in jobs]
+        cu = sum(j.cells * j.steps for j in jobs)
+        cands = self.batch_candidates(jobs)
+        out = {}
+        try:
+            for name, depths in cands.items():
+                for j, T in zip(jobs, depths):
+                    j.temporal_steps = T
+                b = Batch(jobs)
+                b.run()
+                torch.cuda.synchronize()
+                start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                start.record()
+                b.run()
+                stop.record()
+                torch.cuda.synchronize()
+                out[name] = cu / (start.elapsed_time(stop) * 1e-3) / 1e9
+                b.close()
+                del b
+                self.measurements += 1
+        finally:
+            for j, t in zip(jobs, saved):
+                j.temporal_steps = t
+        best = max(out, key=out.get)
+        return cands[best], best, out
 
     def tune(self, rule_name, width, height):
         results = []

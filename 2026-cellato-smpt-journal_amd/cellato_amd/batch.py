@@ -64,6 +64,25 @@ def _init(d, j):
         d.upload(hg.pack(d.rule.index_of(j.initial_values()), d.rule.planes, hg.LINEAR))
 
 
+def plan(jobs, max_waves=0):
+    """The schedule `Batch(jobs)` would build (host only; no GPU needed): start round of each
+    grid's chain of passes, round count, launches, tasks. Every job needs its depth set."""
+    arr = (_lib.BatchGrid * len(jobs))()
+    fake = [0x1000 * (i + 1) for i in range(5)]      # plane pointers are not dereferenced
+    for i, j in enumerate(jobs):
+        g = arr[i]
+        g.rule, g.layout, g.width, g.height = get_rule(j.rule).id, hg.LINEAR, j.width, j.height
+        g.steps, g.t0, g.temporal_steps, g.chunk_rows = j.steps, j.t0, j.temporal_steps, j.chunk_rows
+        for p in range(5):
+            g.a[p] = fake[p]
+            g.b[p] = fake[p] + 0x100000
+    starts = (ctypes.c_int * max(1, len(jobs)))()
+    r, l, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    _lib.check(_lib.lib().cellato_batch_plan(arr, len(jobs), max_waves, starts, ctypes.byref(r), ctypes.byref(l),
+                                             ctypes.byref(t)))
+    return {"start_round": list(starts[:len(jobs)]), "rounds": r.value, "launches": l.value, "tasks": t.value}
+
+
 def _stream():
     import torch
     return torch.cuda.current_stream().cuda_stream
@@ -74,11 +93,18 @@ class Batch:
         self.jobs = list(jobs)
         self.grids = []
         arr = (_lib.BatchGrid * len(self.jobs))()
+        # depths: the job's own, else the tuner's choice measured on the batch itself (4
+        # without a tuner)
+        auto = [j for j in self.jobs if j.temporal_steps <= 0]
+        picked = {}
+        if tuner is not None and auto:
+            depths, _, _ = tuner.batch_depths(auto)
+            picked = {id(j): T for j, T in zip(auto, depths)}
+        self.depths = []
         for i, j in enumerate(self.jobs):
             r = get_rule(j.rule)
-            T, chunk = j.temporal_steps, j.chunk_rows
-            if T <= 0:
-                T, chunk = tuner.pick(j.rule, j.width, j.height) if tuner else (4, 0)
+            T, chunk = (j.temporal_steps, j.chunk_rows) if j.temporal_steps > 0 else (picked.get(id(j), 4), 0)
+            self.depths.append(T)
             d = DeviceGrid(r, j.width, j.height, hg.LINEAR)
             _init(d, j)
             self.grids.append(d)

@@ -3,9 +3,9 @@
 Requests (automaton, size, step count, seed) arrive over time. The server admits them in
 arrival order while the HBM budget and the slot count allow, keeps their grids on the device
 (generated there from the seed), and runs *ticks*: one `cellato_hip_multi` call that advances
-every live job by one launch of its own depth (min(T, steps left) generations). Jobs that
-arrive while others are running join at the next tick; finished jobs leave and free their
-memory. Completion time is the GPU time at which a job's last tick finished (HIP events
+every live job by up to `tick_passes` launches of its own depth T (min(tick_passes x T, steps
+left) generations). Jobs that arrive while others are running join at the next tick;
+finished jobs leave and free their memory. Completion time is the GPU time at which a job's last tick finished (HIP events
 against an event recorded at start), so latency = completion - arrival.
 
 `serve_one_at_a_time` is the comparison point: the same arrivals served in order, each alone
@@ -110,8 +110,9 @@ def admit(waiting, n_live, used, budget, max_live):
 
 class Server:
     def __init__(self, hbm_budget=200e9, max_live=512, tuner=None, default_T=4, keep_results=False,
-                 allocate=None):
+                 allocate=None, tick_passes=4):
         self.budget, self.max_live = hbm_budget, max_live
+        self.tick_passes = tick_passes
         self.tuner, self.default_T = tuner, default_T
         self.keep_results = keep_results
         self.results = {}       # request id -> final DeviceGrid (keep_results)
@@ -156,7 +157,7 @@ class Server:
                 for i, j in enumerate(live):
                     g = arr[i]
                     g.rule, g.layout, g.width, g.height = j.grid.rule.id, 0, j.req.width, j.req.height
-                    g.steps = min(j.T, j.req.steps - j.done)
+                    g.steps = min(j.T * self.tick_passes, j.req.steps - j.done)
                     g.t0, g.temporal_steps, g.chunk_rows = j.done, j.T, j.chunk
                     for p, ptr in enumerate(j.grid.ptrs(j.grid.cur)):
                         g.a[p] = ptr

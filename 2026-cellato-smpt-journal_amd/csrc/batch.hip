@@ -1,9 +1,10 @@
 // Batches of independent grids (different automata, sizes, step counts, depths) advanced by
 // one hipGraph launch. Host planning: every grid's steps are split into passes of its own
-// depth T (the last pass takes the remainder); pass r of every grid forms round r; in each
-// round the passes are grouped by (rule, T) and each group is one persistent batch_kernel
-// launch over the group's task list. Rounds are ordered; the groups of a round run on
-// separate streams. The whole schedule is captured into one graph.
+// depth T (the last pass takes the remainder); a grid's passes occupy consecutive rounds,
+// starting where the rounds' work stays level (rounds.hpp); in each round the passes are
+// grouped by (rule, T) and each group is one persistent batch_kernel launch over the group's
+// task list. Rounds are ordered; the groups of a round run on separate streams. The whole
+// schedule is captured into one graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +17,7 @@
 #include "cellato_hip.h"
 #include "dispatch.hpp"
 #include "plan.hpp"
+#include "rounds.hpp"
 
 namespace cellato::hip {
 namespace {
@@ -25,12 +27,16 @@ int bfail(int code, const std::string& m) {
     return code;
 }
 
-constexpr int SIDE_STREAMS = 8;
+// Streams the groups of a round are spread over in the graph: one per hardware queue of the
+// process (HIP's default is 4). More streams than queues only share queues, where kernels of
+// different streams wait for each other.
+constexpr int SIDE_STREAMS = 4;
 
 struct launch_rec {
     const void* fn;
     int task_begin, task_count;
     int blocks;
+    long long cost;     // rows x generations of the launch, for spreading launches over streams
 };
 
 struct batch {
@@ -71,14 +77,12 @@ struct schedule {
     int launches = 0;
 };
 
-int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_waves) {
+int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_waves, std::vector<int>* starts = nullptr) {
     std::vector<pass_args>& passes = S.passes;
-    // per round: (rule, T) -> list of pass indices
-    std::vector<std::map<std::pair<int, int>, std::vector<int>>> rounds;
-    struct pending_pass { int grid, T, done; };
-    std::vector<std::vector<pending_pass>> plan_T;   // per round: passes to plan
-    std::vector<long long> round_streams;           // per round: total row-streams of work
+    struct pending_pass { int grid, T, done, index; };
+    std::vector<std::vector<pending_pass>> chain(n);   // per grid: its passes in order
     S.result_in_b.assign(n, 0);
+    int R = 0;
     for (int gi = 0; gi < n; ++gi) {
         const cellato_batch_grid& g = grids[gi];
         if (g.rule < 0 || g.rule >= 10) return bfail(CELLATO_E_RULE, "grid " + std::to_string(gi) + ": unknown rule");
@@ -94,46 +98,73 @@ int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_w
         int done = 0, r = 0;
         while (done < g.steps) {
             const int T = std::min(g.temporal_steps, g.steps - done);
-            if (static_cast<int>(plan_T.size()) <= r) { plan_T.resize(r + 1); round_streams.resize(r + 1, 0); }
-            plan_T[r].push_back({gi, T, done});
-            round_streams[r] += pass_streams(g.rule, g.layout, g.width, g.height, T, true);
+            chain[gi].push_back({gi, T, done, r});
             done += T;
             ++r;
         }
         S.result_in_b[gi] = r % 2;
+        R = std::max(R, r);
     }
-    // second pass: plan every pass with the chunk that fills the GPU for its round's total work
-    for (size_t r = 0; r < plan_T.size(); ++r) {
+    // where each grid's chain of passes starts: level the rounds' work (cell updates)
+    std::vector<std::vector<double>> work(n);
+    for (int gi = 0; gi < n; ++gi)
+        for (const auto& e : chain[gi]) work[gi].push_back(static_cast<double>(grids[gi].width) * grids[gi].height * e.T);
+    const std::vector<int> start = place_chains(work, R);
+    if (starts) *starts = start;
+    std::vector<std::vector<pending_pass>> plan_T(R);   // per round: passes to plan
+    // per round and (rule, T) group: total row-streams of work. Each group's launch is sized
+    // to fill the GPU by itself: groups run on few hardware queues and often one after the
+    // other, so a chunk sized for the whole round would leave a lone group's launch short.
+    std::vector<std::map<std::pair<int, int>, long long>> group_streams(R);
+    for (int gi = 0; gi < n; ++gi)
+        for (const auto& e : chain[gi]) {
+            const int r = start[gi] + e.index;
+            const cellato_batch_grid& g = grids[gi];
+            plan_T[r].push_back(e);
+            group_streams[r][{g.rule, e.T}] += pass_streams(g.rule, g.layout, g.width, g.height, e.T, true);
+        }
+    // plan every pass with the chunk that fills the GPU for its group's work
+    std::vector<std::map<std::pair<int, int>, std::vector<int>>> rounds(R);   // (rule, T) -> passes
+    for (int r = 0; r < R; ++r) {
         for (const auto& e : plan_T[r]) {
             const cellato_batch_grid& g = grids[e.grid];
             const int P = kernels_of_rule(g.rule).planes;
             pass_plan pl;
-            const int rc = plan_pass(g.rule, g.layout, g.width, g.height, e.T, true, g.chunk_rows, pl, round_streams[r]);
+            const int rc = plan_pass(g.rule, g.layout, g.width, g.height, e.T, true, g.chunk_rows, pl,
+                                     group_streams[r][{g.rule, e.T}]);
             if (rc != CELLATO_OK) return bfail(rc, plan_error());
+            // pass k of a grid reads a and writes b when k is even, the other way when odd
             for (int p = 0; p < P; ++p) {
-                pl.args.in[p] = r % 2 == 0 ? g.a[p] : g.b[p];
-                pl.args.out[p] = r % 2 == 0 ? g.b[p] : g.a[p];
+                pl.args.in[p] = e.index % 2 == 0 ? g.a[p] : g.b[p];
+                pl.args.out[p] = e.index % 2 == 0 ? g.b[p] : g.a[p];
             }
             pl.args.t0 = g.t0 + e.done;
-            if (rounds.size() <= r) rounds.resize(r + 1);
             rounds[r][{g.rule, e.T}].push_back(static_cast<int>(passes.size()));
             passes.push_back(pl.args);
         }
     }
-    S.rounds.resize(rounds.size());
-    for (size_t r = 0; r < rounds.size(); ++r) {
+    S.rounds.resize(R);
+    for (int r = 0; r < R; ++r) {
+        std::vector<std::pair<long long, launch_rec>> recs;
         for (auto& [key, plist] : rounds[r]) {
             launch_rec L;
             L.fn = kernels_of_rule(key.first).batch[key.second];
             L.task_begin = static_cast<int>(S.tasks.size());
-            for (int pi : plist)
+            long long rows = 0;
+            for (int pi : plist) {
                 for (int t = 0; t < passes[pi].ntasks; ++t) S.tasks.push_back({pi, t});
+                rows += static_cast<long long>(passes[pi].ntasks) * passes[pi].chunk;
+            }
             L.task_count = static_cast<int>(S.tasks.size()) - L.task_begin;
             const int waves = std::min(L.task_count, max_waves);
             L.blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-            S.rounds[r].push_back(L);
+            L.cost = rows * key.second;
+            recs.push_back({L.cost, L});
             ++S.launches;
         }
+        // the launch with the most work first, so that it starts first
+        std::stable_sort(recs.begin(), recs.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        for (auto& rec : recs) S.rounds[r].push_back(rec.second);
     }
     return CELLATO_OK;
 }
@@ -167,9 +198,14 @@ int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
     int status = CELLATO_OK;
     for (auto& round : B.rounds) {
         if (hipEventRecord(B.fork, origin) != hipSuccess) { status = CELLATO_E_HIP; break; }
+        // launches come largest first; each goes to the stream with the least work so far
+        long long load[SIDE_STREAMS] = {};
         for (size_t j = 0; j < round.size() && status == CELLATO_OK; ++j) {
             const launch_rec& L = round[j];
-            hipStream_t s = B.side[j % SIDE_STREAMS];
+            const int k = j < SIDE_STREAMS ? static_cast<int>(j)
+                                           : static_cast<int>(std::min_element(load, load + SIDE_STREAMS) - load);
+            load[k] += L.cost;
+            hipStream_t s = B.side[k];
             if (j < SIDE_STREAMS && hipStreamWaitEvent(s, B.fork, 0) != hipSuccess) { status = CELLATO_E_HIP; break; }
             batch_args args{B.dev_passes, B.dev_tasks + L.task_begin, L.task_count};
             void* kargs[] = {&args};
@@ -195,12 +231,16 @@ int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
 
 // Tables of cellato_hip_multi: two device regions and two pinned host staging buffers used
 // alternately; a region is rewritten only after the kernels that read it have finished.
+// The groups of the call's rounds are spread over SIDE_STREAMS streams forked from and joined
+// back into the caller's stream, as in the graph of a batch.
 struct multi_state {
     void* dev[2] = {nullptr, nullptr};
     void* host[2] = {nullptr, nullptr};
     size_t cap[2] = {0, 0};
     hipEvent_t done[2] = {nullptr, nullptr};
     int next = 0;
+    hipStream_t side[SIDE_STREAMS] = {};
+    hipEvent_t fork = nullptr, join[SIDE_STREAMS] = {};
 };
 
 multi_state& mstate() {
@@ -236,12 +276,37 @@ int run_multi(const cellato_batch_grid* grids, int n, int max_waves, hipStream_t
     BCHECK(hipMemcpyAsync(M.dev[k], M.host[k], bytes, hipMemcpyHostToDevice, stream));
     const auto* dpass = static_cast<const pass_args*>(M.dev[k]);
     const auto* dtask = reinterpret_cast<const batch_task*>(static_cast<char*>(M.dev[k]) + pbytes);
-    for (auto& round : S.rounds)
-        for (const launch_rec& L : round) {
+    if (!M.fork) {
+        for (auto& sd : M.side) BCHECK(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+        for (auto& e : M.join) BCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BCHECK(hipEventCreateWithFlags(&M.fork, hipEventDisableTiming));
+    }
+    for (auto& round : S.rounds) {
+        if (round.size() == 1) {       // one group: no fork
+            const launch_rec& L = round[0];
             batch_args args{dpass, dtask + L.task_begin, L.task_count};
             void* kargs[] = {&args};
             BCHECK(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, stream));
+            continue;
         }
+        BCHECK(hipEventRecord(M.fork, stream));
+        long long load[SIDE_STREAMS] = {};
+        const size_t used = std::min(round.size(), static_cast<size_t>(SIDE_STREAMS));
+        for (size_t j = 0; j < used; ++j) BCHECK(hipStreamWaitEvent(M.side[j], M.fork, 0));
+        for (size_t j = 0; j < round.size(); ++j) {
+            const launch_rec& L = round[j];
+            const int q = j < SIDE_STREAMS ? static_cast<int>(j)
+                                           : static_cast<int>(std::min_element(load, load + SIDE_STREAMS) - load);
+            load[q] += L.cost;
+            batch_args args{dpass, dtask + L.task_begin, L.task_count};
+            void* kargs[] = {&args};
+            BCHECK(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, M.side[q]));
+        }
+        for (size_t j = 0; j < used; ++j) {
+            BCHECK(hipEventRecord(M.join[j], M.side[j]));
+            BCHECK(hipStreamWaitEvent(stream, M.join[j], 0));
+        }
+    }
     BCHECK(hipEventRecord(M.done[k], stream));
     return CELLATO_OK;
 }
@@ -290,6 +355,21 @@ int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tas
 int cellato_hip_multi(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b) {
     if ((n > 0 && (!grids || !result_in_b)) || n < 0) return bfail(CELLATO_E_ARG, "null argument");
     return run_multi(grids, n, max_waves > 0 ? max_waves : 8192, static_cast<hipStream_t>(stream), result_in_b);
+}
+
+int cellato_batch_plan(const cellato_batch_grid* grids, int n, int max_waves, int* start_round, int* rounds,
+                       int* launches, long long* tasks) {
+    if ((n > 0 && !grids) || n < 0) return bfail(CELLATO_E_ARG, "null argument");
+    schedule S;
+    std::vector<int> starts;
+    const int rc = plan_schedule(S, grids, n, max_waves > 0 ? max_waves : 8192, &starts);
+    if (rc != CELLATO_OK) return rc;
+    if (start_round)
+        for (int i = 0; i < n; ++i) start_round[i] = starts[i];
+    if (rounds) *rounds = static_cast<int>(S.rounds.size());
+    if (launches) *launches = S.launches;
+    if (tasks) *tasks = static_cast<long long>(S.tasks.size());
+    return CELLATO_OK;
 }
 
 int cellato_batch_destroy(void* handle) {

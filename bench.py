@@ -50,6 +50,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the 2048^2 bit-exactness check")
     ap.add_argument("--no-simple", action="store_true", help="measure the temporal depths only")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--batch-grids", type=int, default=512)
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--tick-passes", type=int, default=16, help="passes per live job and serving tick")
+    ap.add_argument("--serve-baseline-limit", type=float, default=45.0,
+                    help="seconds after which the one-at-a-time baseline stops admitting requests")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also time a plain device copy of this many MiB (the calibration kernel for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -116,6 +124,137 @@ def cpu_baseline(C, O):
             "all_cores": {"value": round(many, 3), "threads": threads, "host_cpus": cores,
                           "sample": "same grid, 16 generations, rows split over threads"},
             "standard_evaluator_1core": {"value": round(std, 4), "sample": "1024^2, 4 generations"}}
+
+
+def make_tuner():
+    """Depth tuner guided by the cost model, trained on the GPU on the committed MI355X
+    dataset; measures the model's two best depths per (automaton, size) bucket."""
+    import numpy as np
+    from cellato_amd.autotune import Tuner, features
+    from cellato_amd.costmodel import CostModel
+    from cellato_amd.tuning_eval import load
+    rows, _ = load()
+    X = np.array([features(r["rule"], r["width"], r["height"], r["T"]) for r in rows], np.float32)
+    y = np.log(np.array([r["ns_per_cell"] for r in rows], np.float32))
+    t = time.perf_counter()
+    model = CostModel(X.shape[1], hidden=(64, 64), seed=0)
+    model.fit(X, y, epochs=600)
+    train_s = time.perf_counter() - t
+    return Tuner(model=model, budget=2, target_cells=1 << 26), train_s
+
+
+def bench_batch(C, O, torch, tuner, n):
+    """Batching configuration: n grids of five automata, sides log-uniform 64..4096, steps
+    16..1024, one graph launch; against the plain per-grid launch loop."""
+    import numpy as np
+    from cellato_amd.batch import Batch, PlainLoop, batch_workload
+    from cellato_amd.rules import device_random_values
+    jobs = batch_workload(n, seed=2026)
+    cu = sum(j.cells * j.steps for j in jobs)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        fn()
+        e.record()
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / 1e3
+
+    t = time.perf_counter()
+    depths, winner, measured = tuner.batch_depths(jobs)
+    tune_s = time.perf_counter() - t
+    for j, T in zip(jobs, depths):
+        j.temporal_steps = T
+    b = Batch(jobs)
+    b.run()                          # warm-up replay, then back to the initial states
+    b.reset()
+    batch_s = timed(b.run)
+    stats = b.stats()
+    sums_batch = [b.checksum(i) for i in range(len(jobs))]
+    small = sorted(range(len(jobs)), key=lambda i: jobs[i].cells * jobs[i].steps)[:8]
+    oracle_ok = all(
+        np.array_equal(b.result(i), O.run_standard(jobs[i].rule,
+                       device_random_values(jobs[i].rule, jobs[i].width, jobs[i].height, jobs[i].seed), jobs[i].steps))
+        for i in small)
+    b.close()
+    del b
+    torch.cuda.empty_cache()
+    pl = PlainLoop(jobs)
+    plain_s = timed(pl.run)
+    sums_plain = [pl.grids[i].checksum_device() for i in range(len(jobs))]
+    pl.reset()
+    for j in jobs:                   # the plain loop with each grid's tuned depth
+        tuner.pick(j.rule, j.width, j.height)
+    plain_t_s = timed(lambda: pl.run(tuner=tuner))
+    del pl
+    torch.cuda.empty_cache()
+    return {"workload": f"{n} grids, automata uniform over game-of-life/brian/forest-fire/traffic/maze, square sides "
+                        "log-uniform 64..4096 (multiple of 64), steps uniform 16..1024, seeded i.i.d. states made on "
+                        "the device, one seed",
+            "value": round(cu / batch_s / 1e9, 1), "unit": "GCUPS", "cell_updates": cu,
+            "ms": round(batch_s * 1e3, 3), "graph_launches": 1, **stats,
+            "depths": winner,
+            "depths_per_automaton": {k: sorted({T for j, T in zip(jobs, depths) if j.rule == k})
+                                     for k in sorted({j.rule for j in jobs})},
+            "depth_candidates_gcups": {k: round(v, 1) for k, v in measured.items()},
+            "plain_loop_gcups": round(cu / plain_s / 1e9, 1),
+            "speedup_vs_plain_loop": round(plain_s / batch_s, 2),
+            "plain_loop_tuned_depth_gcups": round(cu / plain_t_s / 1e9, 1),
+            "bit_exact_vs_plain_all_grids": sums_batch == sums_plain,
+            "bit_exact_vs_cpu_oracle_8_smallest": bool(oracle_ok),
+            "tuning_seconds": round(tune_s, 2)}
+
+
+def bench_serve(C, O, torch, tuner, n, rate, limit, tick_passes):
+    """Serving configuration: Poisson arrivals, all ten automata, sides 128..8192, steps
+    64..2048, continuous batching under a 200 GB HBM budget; against serving one request at
+    a time with the plain simple traverser. Every result is compared with the one-at-a-time
+    result by on-device checksums, and the four smallest with the CPU oracle."""
+    import numpy as np
+    from cellato_amd.serve import Server, serving_workload, serve_one_at_a_time
+    from cellato_amd.rules import device_random_values
+    reqs = serving_workload(n=n, rate=rate, seed=7)
+    srv = Server(hbm_budget=200e9, default_T=4, tick_passes=tick_passes, keep_results=True)
+    rep = srv.run(reqs)
+    s = rep.summary()
+    sample = sorted(reqs, key=lambda r: r.cell_updates)[:4]
+    exact = all(np.array_equal(srv.results[r.id].to_plane_grid().to_standard(),
+                               O.run_standard(r.rule, device_random_values(r.rule, r.width, r.height, r.seed), r.steps))
+                for r in sample)
+    sums = {i: g.checksum_device() for i, g in srv.results.items()}
+    srv.results.clear()
+    torch.cuda.empty_cache()
+    for r in reqs:                   # tune outside the served time
+        tuner.pick(r.rule, r.width, r.height)
+    tuned = Server(hbm_budget=200e9, tuner=tuner, tick_passes=tick_passes).run(reqs).summary()
+    torch.cuda.empty_cache()
+    base, base_results = serve_one_at_a_time(reqs, keep_results=True, time_limit=limit)
+    same = all(sums[i] == g.checksum_device() for i, g in base_results.items())
+    base_results.clear()
+    torch.cuda.empty_cache()
+    b = base.summary()
+    done = len(base.latencies)
+    # requests the baseline did not reach wait at least until it stopped
+    first = min(r.arrival for r in reqs)
+    censored = [max(0.0, base.makespan + first - r.arrival) for r in reqs if r.id not in base.latencies]
+    lat_all = np.array(list(base.latencies.values()) + censored)
+    return {"workload": f"{n} requests, Poisson {rate:g}/s, automaton uniform over the ten, square side log-uniform "
+                        "128..8192, steps uniform 64..2048, seeded i.i.d. states made on the device, HBM budget 200 GB",
+            "p50_s": round(s["p50_s"], 4), "p99_s": round(s["p99_s"], 4), "gcups": round(s["gcups"], 1),
+            "makespan_s": round(s["makespan_s"], 3), "temporal_steps": 4, "tick_passes": tick_passes,
+            "ticks": s["ticks"], "max_live": s["max_live"], "peak_GB": round(s["peak_GB"], 2),
+            "bit_exact_vs_one_at_a_time": bool(same), "compared": len(sums) if done == n else done,
+            "bit_exact_vs_cpu_oracle_4_smallest": bool(exact),
+            "tuned_depth_per_automaton_and_size": {"gcups": round(tuned["gcups"], 1), "p50_s": round(tuned["p50_s"], 4),
+                                                   "p99_s": round(tuned["p99_s"], 4)},
+            "one_at_a_time": {"completed": done, "of": n, "gcups": round(b["gcups"], 1),
+                              "p50_s": round(float(np.percentile(lat_all, 50)), 4),
+                              "p99_s": round(float(np.percentile(lat_all, 99)), 4),
+                              "latency_note": "requests not served within the time limit count with their "
+                                              "waiting time so far (a lower bound)" if done < n else ""},
+            "gcups_vs_one_at_a_time": round(s["gcups"] / b["gcups"], 2) if b["gcups"] else None,
+            "p99_vs_one_at_a_time": round(float(np.percentile(lat_all, 99)) / s["p99_s"], 2) if s["p99_s"] else None}
 
 
 def main():
@@ -203,6 +342,18 @@ def main():
             "unit": "GB/s", "frac": round(simple_gol["algorithmic_GBps"] / HBM_PEAK_GBPS, 4)},
         "variants": variants,
     }
+    if not (a.no_batch and a.no_serve):
+        tuner, train_s = make_tuner()
+        out["tuner"] = {"model": "MLP 15-64-64-1 trained on the GPU (csrc/mlp.hip) on "
+                                 "cellato_amd/data/depth_tuning_mi355x.json", "train_seconds": round(train_s, 2),
+                        "measured_per_bucket": tuner.budget}
+        if not a.no_batch:
+            out["batched"] = bench_batch(C, O, torch, tuner, a.batch_grids)
+            print(json.dumps({"batched": out["batched"]}), file=sys.stderr, flush=True)
+        if not a.no_serve:
+            out["served"] = bench_serve(C, O, torch, tuner, a.serve_requests, a.serve_rate, a.serve_baseline_limit,
+                                        a.tick_passes)
+            print(json.dumps({"served": out["served"]}), file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(C, O)
     print(json.dumps(out), flush=True)

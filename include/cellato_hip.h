@@ -99,6 +99,11 @@ int cellato_batch_launch(void* handle, void* stream);
 int cellato_batch_result_in_b(void* handle, int grid);
 int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tasks);
 int cellato_batch_destroy(void* handle);
+/* Schedule of a batch without building it (host only, no GPU work; the plane pointers must
+ * be non-null but are not dereferenced): the round in which each grid's chain of passes starts (start_round, n entries, may
+ * be NULL), the round count, kernel launches and tasks. */
+int cellato_batch_plan(const cellato_batch_grid* grids, int n, int max_waves, int* start_round, int* rounds,
+                       int* launches, long long* tasks);
 
 /* The same schedule launched directly (no graph) on `stream`, for callers whose set of grids
  * changes between calls (the serving loop calls it once per tick with one pass per live job).
