@@ -24,6 +24,14 @@ class BatchGrid(ctypes.Structure):
                 ("chunk_rows", ctypes.c_int), ("a", ctypes.c_void_p * 5), ("b", ctypes.c_void_p * 5)]
 
 
+def plan(rule_id, layout, width, height, temporal_steps=0, chunk_rows=0):
+    """The launch one pass would make (host only): (rows per task, tasks, blocks)."""
+    c, t, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().cellato_hip_plan(rule_id, layout, width, height, temporal_steps, chunk_rows, ctypes.byref(c),
+                                 ctypes.byref(t), ctypes.byref(b)))
+    return c.value, t.value, b.value
+
+
 class CellatoError(RuntimeError):
     """A failed C-ABI call: `code` is the negative status, the message comes from the library."""
 
@@ -72,6 +80,7 @@ def lib():
         L.cellato_batch_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_longlong)]
         L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.cellato_hip_plan.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int)] * 3
         L.cellato_batch_plan.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]

@@ -151,6 +151,24 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
                       static_cast<hipStream_t>(stream), result_in_b);
 }
 
+int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_steps, int chunk_rows, int* chunk,
+                     int* tasks, int* blocks) {
+    if (rule < 0 || rule >= RULES) return fail(CELLATO_E_RULE, "unknown rule id " + std::to_string(rule));
+    if (layout != CELLATO_LAYOUT_LINEAR && layout != CELLATO_LAYOUT_TILED) return fail(CELLATO_E_LAYOUT, "unknown layout");
+    if (width <= 0 || height <= 0 || (layout == CELLATO_LAYOUT_LINEAR && width % 64) ||
+        (layout == CELLATO_LAYOUT_TILED && (width % 8 || height % 8)))
+        return fail(CELLATO_E_GEOMETRY, "grid size not representable in the layout");
+    if (temporal_steps < 0 || temporal_steps > MAX_T) return fail(CELLATO_E_TEMPORAL, "temporal_steps out of range");
+    pass_plan pl;
+    const bool temporal = temporal_steps > 0;
+    const int rc = plan_pass(rule, layout, width, height, temporal ? temporal_steps : 1, temporal, chunk_rows, pl);
+    if (rc != CELLATO_OK) return fail(rc, plan_error());
+    if (chunk) *chunk = pl.args.chunk;
+    if (tasks) *tasks = pl.args.ntasks;
+    if (blocks) *blocks = pl.blocks;
+    return CELLATO_OK;
+}
+
 const char* cellato_hip_last_error(void) { return last_error().c_str(); }
 
 } // extern "C"

@@ -3,6 +3,8 @@
 #ifndef CELLATO_HIP_PLAN_HPP
 #define CELLATO_HIP_PLAN_HPP
 
+#include <hip/hip_runtime.h>
+
 #include <string>
 
 #include "cellato_hip.h"
@@ -14,6 +16,7 @@ struct pass_plan {
     const void* fn = nullptr;
     pass_args args{};
     int blocks = 0;
+    long long streams = 0;   // rows x lane groups of work (the unit auto_chunk divides)
 };
 
 inline std::string& plan_error() {
@@ -27,20 +30,50 @@ inline std::string& last_error() {
     return e;
 }
 
-// Rows per task when the caller passes 0. A task streams `chunk` rows plus 2T rows of
-// pipeline fill and drain, so long chunks waste less; but a launch needs a few thousand
-// waves to keep 1024 SIMDs busy. Pick the longest chunk that still yields about
-// TARGET_TASKS tasks for `streams` row-streams of work (segments x rows over all grids of
-// the launch), never shorter than max(8, 2T) and never longer than the grid.
-constexpr int TARGET_TASKS = 3072;
+// Compute units of the current device (256 on MI355X; also the answer without a device, so
+// that plans made on a host without a GPU match).
+inline int device_cus() {
+    static const int cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n <= 0) {
+            (void)hipGetLastError();
+            n = 256;
+        }
+        return n;
+    }();
+    return cus;
+}
 
-inline int auto_chunk(long long streams, int T, int rows_per_group) {
-    long long c = (streams + TARGET_TASKS - 1) / TARGET_TASKS;
-    const int lo = 2 * T > 8 ? 2 * T : 8;
-    if (c < lo) c = lo;
-    if (c > 256) c = 256;
-    if (c > rows_per_group) c = rows_per_group > 0 ? rows_per_group : 1;
-    return static_cast<int>(c);
+// Rows per task when the caller passes 0. Every wave runs one task, and a CU takes its
+// blocks (4 waves, one per SIMD) at once, so a launch lasts about m x (rows a task walks),
+// m = blocks on the busiest CU; a task walks chunk + 2T rows rounded up to whole unrolled
+// blocks of unroll_rows(T) rows. For m = 3 and m = 2 the smallest chunk (at least max(8, 2T),
+// filling its last unrolled block) whose tasks fit in m blocks per CU is found, and the one
+// with fewer rows per SIMD wins; ties keep m = 3 (more waves hide more latency). One block
+// per CU leaves loads exposed for light rules and is never chosen unless the grid is small.
+// tasks_of(chunk) = the launch's task count with that chunk.
+template <class TasksOf>
+inline int auto_chunk(TasksOf tasks_of, int T, int rows_per_group) {
+    const int U = unroll_rows(T);
+    const auto align = [&](long long c) { return (c + 2 * T + U - 1) / U * U - 2 * T; };
+    const long long lo = align(2 * T > 8 ? 2 * T : 8);
+    const long long hi = rows_per_group > 0 ? rows_per_group : 1;
+    const long long cus = device_cus();
+    long long best = -1, best_cost = 0;
+    for (int m = 3; m >= 2; --m) {
+        long long c = lo;
+        while (c < hi && (tasks_of(static_cast<int>(c)) + 3) / 4 > m * cus) c += U;
+        if (c > hi) c = hi;
+        const long long blocks = (tasks_of(static_cast<int>(c)) + 3) / 4;
+        const long long per_cu = (blocks + cus - 1) / cus;
+        const long long cost = per_cu * ((c + 2 * T + U - 1) / U * U);
+        if (best < 0 || cost < best_cost) {
+            best = c;
+            best_cost = cost;
+        }
+    }
+    return static_cast<int>(best);
 }
 
 // width / height in cells. T = generations per launch (1 for the simple traverser).
@@ -56,6 +89,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
     }
     a.wx = wx;
     a.h = h;
+    a.tail_n = a.tail_groups = a.tail_first = a.first = 0;
     if (wx < 64) {
         a.n = wx;
         a.groups = 64 / wx;
@@ -78,10 +112,19 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         a.variant = -1;   // halo loads: not used by the batch kernel
         pl.fn = k.simple[layout][SIMPLE_SEGMENT];
     } else {
+        // full segments of 62 owned columns; the columns left over (fewer than 62) go to
+        // narrow lane groups, several row bands per wave, instead of a mostly idle wave
         a.n = 64;
         a.groups = 1;
         a.stride = 62;
-        a.nseg = (wx + 61) / 62;
+        a.first = -1;
+        a.nseg = wx / 62;
+        const int left = wx - 62 * a.nseg;
+        if (left > 0) {
+            a.tail_n = left + 2;
+            a.tail_groups = 64 / a.tail_n;
+            a.tail_first = 62 * a.nseg - 1;
+        }
         a.variant = VARIANT_SEGMENT;
         pl.fn = k.temporal[layout][TEMPORAL_SEGMENT][T];
     }
@@ -90,15 +133,25 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         return CELLATO_E_TEMPORAL;
     }
     const int rows_per_group = (h + a.groups - 1) / a.groups;
-    const long long streams = launch_streams > 0 ? launch_streams : static_cast<long long>(a.nseg) * rows_per_group;
-    a.chunk = chunk > 0 ? chunk : auto_chunk(streams, T, rows_per_group);
+    pl.streams = static_cast<long long>(a.nseg) * rows_per_group +
+                 (a.tail_n > 0 ? (h + a.tail_groups - 1) / a.tail_groups : 0);
+    // tasks with a given chunk: this pass alone, or a share of a launch of `launch_streams`
+    const auto tasks_of = [&](int c) -> long long {
+        if (launch_streams > 0) return (launch_streams + c - 1) / c;
+        const long long nch = (h + static_cast<long long>(a.groups) * c - 1) / (static_cast<long long>(a.groups) * c);
+        return a.nseg * nch + (a.tail_n > 0 ? (h + static_cast<long long>(a.tail_groups) * c - 1) /
+                                                   (static_cast<long long>(a.tail_groups) * c) : 0);
+    };
+    a.chunk = chunk > 0 ? chunk : auto_chunk(tasks_of, T, rows_per_group);
     a.nchunk = (h + a.groups * a.chunk - 1) / (a.groups * a.chunk);
-    const long long tasks = static_cast<long long>(a.nseg) * a.nchunk;
+    const long long main_tasks = static_cast<long long>(a.nseg) * a.nchunk;
+    const long long tasks = main_tasks + (a.tail_n > 0 ? (h + a.tail_groups * a.chunk - 1) / (a.tail_groups * a.chunk) : 0);
     if (tasks > (1LL << 30)) {
         plan_error() = "grid too large for one launch";
         return CELLATO_E_GEOMETRY;
     }
     a.ntasks = static_cast<int>(tasks);
+    a.main_tasks = static_cast<int>(main_tasks);
     pl.blocks = (a.ntasks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
     a.t0 = 0;
     return CELLATO_OK;
@@ -108,7 +161,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
 inline long long pass_streams(int rule, int layout, int width, int height, int T, bool temporal) {
     pass_plan pl;
     if (plan_pass(rule, layout, width, height, T, temporal, 1, pl) != CELLATO_OK) return 0;
-    return static_cast<long long>(pl.args.nseg) * ((pl.args.h + pl.args.groups - 1) / pl.args.groups);
+    return pl.streams;
 }
 
 } // namespace cellato::hip

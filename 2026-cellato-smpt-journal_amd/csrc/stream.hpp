@@ -40,7 +40,11 @@ enum xch_kind : int {
 enum halo_kind : int {
     HALO_NONE = 0,     // rings: no outside
     HALO_LOAD = 1,     // the end lanes load the outside column (exact for one generation)
-    HALO_LANES = 2,    // the end lanes are halo lanes: computed from garbage, never stored
+    HALO_LANES = 2,    // lane groups whose end lanes are halo lanes: computed from garbage,
+                       // never stored. A row is cut into 62-column segments (one 64-lane group
+                       // per wave) plus a tail of the columns left over (groups of columns + 2
+                       // lanes, each group another band of rows); a wave shift across a group
+                       // boundary reaches only halo lanes.
 };
 
 // Kernel arguments of one grid pass (one launch over one grid).
@@ -56,7 +60,12 @@ struct pass_args {
     int stride;      // first column of segment s is s*stride - (halo lanes ? 1 : 0)
     int nseg;        // segments per row
     int nchunk;      // row chunks
-    int ntasks;      // nseg * nchunk
+    int ntasks;      // main_tasks + the tail's tasks
+    int main_tasks;  // nseg * nchunk; tasks from here on cover the tail columns
+    int first;       // halo-lane groups: column of lane 0 of segment 0 (a halo column)
+    int tail_first;  // halo-lane segments: first column (a halo column) of the tail groups
+    int tail_n;      // lanes per tail group (tail columns + 2), 0 = no tail
+    int tail_groups; // tail groups per wave
     int variant;     // lane mapping, for the batch kernel: 0 segments + halo lanes, 1 ring of 64, 2 groups
 };
 
@@ -287,16 +296,16 @@ __device__ __forceinline__ void buf_store(word v, __amdgpu_buffer_rsrc_t r, uint
 
 // Rows per unrolled loop body: a multiple of 3 so that the 3-row windows rotate by renaming
 // registers instead of moving them; also the prefetch distance in rows.
-template <int T>
-constexpr int unroll_rows() { return T <= 2 ? 6 : 3; }
+constexpr int unroll_rows(int T) { return T <= 2 ? 6 : 3; }
 
 // ---- one task: a band of columns x a chunk of rows, T generations --------------------------
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     constexpr int P = Eval::planes;
     constexpr bool LOADS_HALO = HALO == HALO_LOAD;
-    constexpr bool GROUPED = X == XCH_PERMUTE;   // lane groups walk different rows
-    constexpr int U = unroll_rows<T>();
+    // lane groups walk different rows: rows narrower than a wave, and the halo-lane tail
+    constexpr bool GROUPED = X == XCH_PERMUTE || HALO == HALO_LANES;
+    constexpr int U = unroll_rows(T);
     static_assert(!(HALO == HALO_LOAD && T != 1), "loaded halos are exact for one generation only");
 
     const int lane = lane_id();
@@ -313,11 +322,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     bool end_lane = false;
     xch<X> xc{};
     if constexpr (HALO == HALO_LANES) {
-        // segments overlap by two columns; lanes 0 and 63 are halo lanes
-        const int first = seg * a.stride - 1;
-        col = wrap(first + lane, a.wx);
-        const int owned = seg * a.stride + lane - 1;
-        owner = lane >= 1 && lane <= 62 && owned < a.wx;
+        // groups overlap their neighbours by two columns; lanes 0 and n-1 of a group are halo lanes
+        col = wrap(a.first + seg * a.stride + w, a.wx);
+        owner = lane_used && w >= 1 && w <= a.n - 2;
     } else if constexpr (HALO == HALO_LOAD) {
         // full 64-column segments; the last one is moved back to end at the row's end
         // (columns it shares with its neighbour are written twice with the same value)
@@ -479,12 +486,33 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
     return xcd < rem ? xcd * (q + 1) + local : rem * (q + 1) + (xcd - rem) * q + local;
 }
 
+// One task of a pass whose rows are cut into 62-column halo-lane segments plus a tail of
+// narrow lane groups. Both kinds run the same loop (one copy of the code): only the lane
+// mapping differs, and it is set up here, uniformly across the wave.
+template <class Eval, int LAYOUT, int T>
+__device__ __forceinline__ void segmented_task(const pass_args& a, int task) {
+    pass_args b = a;
+    int t = task;
+    if (task >= a.main_tasks) {
+        b.n = a.tail_n;
+        b.groups = a.tail_groups;
+        b.first = a.tail_first;
+        b.stride = 0;
+        b.nseg = 1;
+        t = task - a.main_tasks;
+    }
+    stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES>(b, t);
+}
+
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
     const int blk = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int task = __builtin_amdgcn_readfirstlane(blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
     if (task >= a.ntasks) return;
-    stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
+    if constexpr (HALO == HALO_LANES)
+        segmented_task<Eval, LAYOUT, T>(a, task);
+    else
+        stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
 }
 
 // ---- batches: many grids, one persistent launch per (rule, T) and round ---------------------
@@ -510,7 +538,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) batch_kernel(batch_args b) {
         const batch_task t = b.tasks[i];
         const pass_args& a = b.passes[t.pass];
         if (a.variant == VARIANT_SEGMENT)
-            stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES>(a, t.task);
+            segmented_task<Eval, LAYOUT, T>(a, t.task);
         else if (a.variant == VARIANT_RING64)
             stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE>(a, t.task);
         else

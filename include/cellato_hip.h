@@ -73,6 +73,13 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
                          int width, int height, int t0, int steps, int temporal_steps,
                          int chunk_rows, void* stream, int* result_in_b);
 
+/* The launch one pass would make (host only, no GPU work): rows per task (`chunk`; chunk_rows
+ * 0 = the automatic choice), tasks (one wave each) and blocks of 4 waves. temporal_steps 0 =
+ * the simple traverser's kernel, 1..max = the temporal traverser's. No counterpart in the
+ * reference (whose launch shape is fixed by its tile-size parameters). */
+int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_steps, int chunk_rows, int* chunk,
+                     int* tasks, int* blocks);
+
 /* ---- batches ------------------------------------------------------------------------------
  * Many independent grids (any of the rules, any size, own step count, start step and depth)
  * advanced together: the steps of every grid are cut into launches of its own depth, the

@@ -71,3 +71,31 @@ def test_temporal_depth_is_checked():
 def test_zero_steps_launch_nothing():
     assert _lib.simple(0, 0, FAKE, FAKE, 128, 16, 0, 0) is False
     assert _lib.temporal(9, 1, FAKE, FAKE, 64, 16, 0, 0, 4) is False
+
+
+@pytest.mark.parametrize("T", [0, 1, 2, 4, 8])
+def test_plan_fills_every_simd_in_whole_unrolled_blocks(T):
+    # automatic rows per task at 16384^2: every task walks whole unrolled blocks of rows, the
+    # tasks fill 2 or 3 blocks of 4 waves per CU (256 CUs), and with halo-lane segments the
+    # 8 columns left over after four 62-column segments go to groups of 10 lanes, 6 per wave
+    import math
+    chunk, tasks, blocks = _lib.plan(0, 0, 16384, 16384, T)
+    U = 6 if T <= 2 else 3
+    steps = chunk + 2 * max(T, 1)
+    assert steps % U == 0
+    assert blocks == math.ceil(tasks / 4)
+    assert math.ceil(blocks / 256) in (2, 3)
+    if T >= 1:
+        assert tasks == 4 * math.ceil(16384 / chunk) + math.ceil(16384 / (6 * chunk))
+    else:
+        assert tasks == 4 * math.ceil(16384 / chunk)
+    # an explicit chunk is taken as given
+    assert _lib.plan(0, 0, 16384, 16384, T, 7)[0] == 7
+
+
+def test_plan_rejects_what_the_traversers_reject():
+    for args, code in [((11, 0, 128, 16, 0), _lib.E_RULE), ((0, 3, 128, 16, 0), _lib.E_LAYOUT),
+                       ((0, 0, 100, 16, 0), _lib.E_GEOMETRY), ((0, 0, 128, 16, 9), _lib.E_TEMPORAL)]:
+        with pytest.raises(_lib.CellatoError) as e:
+            _lib.plan(*args)
+        assert e.value.code == code

@@ -15,7 +15,7 @@ from gpu_util import run_gpu
 pytestmark = pytest.mark.gpu
 
 RULES = ["game-of-life", "wire", "forest-fire", "excitable", "brian", "maze", "traffic", "fluid", "critters", "cyclic"]
-LINEAR_SHAPES = [(192, 70), (4096, 40), (8192, 50), (8320, 33), (4160, 20), (128, 1), (64, 3)]
+LINEAR_SHAPES = [(192, 70), (4096, 40), (8192, 50), (8320, 33), (4160, 20), (16384, 24), (7936, 12), (128, 1), (64, 3)]
 TILED_SHAPES = [(40, 24), (512, 16), (1024, 24), (1040, 32), (520, 40), (8, 8)]
 TRAVERSERS = [("simple", 1), ("temporal", 1), ("temporal", 2), ("temporal", 3), ("temporal", 5), ("temporal", 8)]
 
