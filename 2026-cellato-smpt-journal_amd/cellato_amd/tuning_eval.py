@@ -28,7 +28,7 @@ def load(path=DATA):
 def evaluate(budget=2, epochs=600, seed=0, hidden=(64, 64), path=DATA, holdout="rule"):
     from .costmodel import CostModel
     rows, table = load(path)
-    part = (lambda rule, w: rule) if holdout == "rule" else (lambda rule, w: w)
+    part = (lambda rule, w: rule) if holdout in ("rule", "automaton") else (lambda rule, w: w)
     folds = sorted({part(k[0], k[1]) for k in table})
     rng = np.random.default_rng(seed)
     res = {"model_topk": [], "model_top1": [], "default_T4": [], "random_k": []}

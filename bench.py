@@ -22,19 +22,21 @@ sys.path.insert(0, str(REPO / "2026-cellato-smpt-journal_amd"))
 sys.path.insert(0, str(REPO / "tests"))
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated against a
-# plain copy kernel); see profiles/stream_kernel_gol_cyclic_16384_hbm_traffic.md
+# HBM bytes and VALU instructions per launch measured with rocprofv3 (FETCH_SIZE / WRITE_SIZE
+# calibrated against a plain copy kernel, SQ_INSTS_VALU); see profiles/stream_kernel_16384_*.md
 TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_16384.json"
+# VALU issue peak: one wave64 instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs, 2.4 GHz
+VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def measured_traffic(automaton, traverser, T, size):
+def measured_launch(automaton, traverser, T, size, key="traffic_bytes"):
     try:
         data = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
     for e in data["entries"]:
         if (e["automaton"], e["traverser"], e["temporal_steps"], e["size"]) == (automaton, traverser, T, size):
-            return int(e["traffic_bytes"])
+            return int(e[key]) if key in e else None
     return None
 
 
@@ -257,6 +259,19 @@ def bench_serve(C, O, torch, tuner, n, rate, limit, tick_passes):
             "p99_vs_one_at_a_time": round(float(np.percentile(lat_all, 99)) / s["p99_s"], 2) if s["p99_s"] else None}
 
 
+def valu_roofline(g, size):
+    """Temporal launches are bound by VALU issue, not HBM: wave64 VALU instructions per launch
+    (rocprofv3 SQ_INSTS_VALU, a fixed count for the configuration) over the live average
+    launch duration, against the issue peak."""
+    ins = measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], size, "valu_wave_instructions")
+    if not ins:
+        return None
+    rate = ins / (g["ms_per_launch"] * 1e-3)
+    return {"achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_INSTR_PER_S / 1e9, 1),
+            "unit": "G wave64 VALU instructions/s", "frac": round(rate / VALU_PEAK_INSTR_PER_S, 4),
+            "instructions_per_launch": ins}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -334,9 +349,10 @@ def main():
         "bit_exact": all(r.get("bit_exact_2048_subset", True) for r in variants),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_traffic(g["automaton"], g["traverser"], g["temporal_steps"], a.size),
+                     "traffic": measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], a.size),
                      "kernel": f"stream_kernel ({g['traverser']}, T={g['temporal_steps']})",
-                     "bytes_per_launch": 2 * C.rule(g["automaton"]).planes * a.size * a.size // 8},
+                     "bytes_per_launch": 2 * C.rule(g["automaton"]).planes * a.size * a.size // 8,
+                     "valu": valu_roofline(g, a.size)},
         "roofline_simple": None if simple_gol is None else {
             "bound": "hbm", "achieved": simple_gol["algorithmic_GBps"], "peak": HBM_PEAK_GBPS,
             "unit": "GB/s", "frac": round(simple_gol["algorithmic_GBps"] / HBM_PEAK_GBPS, 4)},
@@ -344,9 +360,13 @@ def main():
     }
     if not (a.no_batch and a.no_serve):
         tuner, train_s = make_tuner()
+        from cellato_amd.tuning_eval import evaluate
+        gains = {h: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in evaluate(budget=2, holdout=h).items()}
+                 for h in ("size", "automaton")}
         out["tuner"] = {"model": "MLP 15-64-64-1 trained on the GPU (csrc/mlp.hip) on "
                                  "cellato_amd/data/depth_tuning_mi355x.json", "train_seconds": round(train_s, 2),
-                        "measured_per_bucket": tuner.budget}
+                        "measured_per_bucket": tuner.budget,
+                        "held_out_fraction_of_best": gains}
         if not a.no_batch:
             out["batched"] = bench_batch(C, O, torch, tuner, a.batch_grids)
             print(json.dumps({"batched": out["batched"]}), file=sys.stderr, flush=True)
