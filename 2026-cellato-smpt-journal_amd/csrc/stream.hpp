@@ -21,6 +21,8 @@
 namespace cellato::hip {
 
 using word = std::uint64_t;
+using cellato::evaluators::bit_planes::klist;
+using cellato::evaluators::bit_planes::kindex;
 using cellato::evaluators::bit_planes::vint;
 
 constexpr int MAX_PLANES = 5;
@@ -146,12 +148,17 @@ using row_t = word[P];
 // A row of the linear layout as held in a ring slot: the P plane words of this lane's column
 // plus the edge halves of the neighbouring columns that the 1-cell shifts need (bits 32..63
 // of the column to the left, bits 0..31 of the column to the right), fetched from the
-// neighbouring lanes once when the row enters the ring.
-template <int P>
+// neighbouring lanes once when the row enters the ring; and, for each of the NS states the
+// rule counts over the Moore neighbourhood, the row's 3-cell horizontal sums of that
+// state's mask (sum bit, carry), also formed once at entry and read by the three output rows
+// that use the row.
+template <int P, int NS = 0>
 struct lrow {
     word w[P];
     uint32_t l[P];   // hi half of the left column's word
     uint32_t r[P];   // lo half of the right column's word
+    word rs_s[NS > 0 ? NS : 1];
+    word rs_c[NS > 0 ? NS : 1];
 };
 
 // the word of a cell row shifted by one cell: dx = -1 gives, at bit i, the cell at x - 1
@@ -166,19 +173,30 @@ __device__ __forceinline__ word shift1(word own, uint32_t left_hi, uint32_t righ
     }
 }
 
-// Linear layout: a word is 64 cells of one row; cell x at bit x % 64.
-template <int P>
+// Linear layout: a word is 64 cells of one row; cell x at bit x % 64. Targets = the counted
+// states whose row sums the ring slots carry.
+template <int P, class Targets = klist<>>
 struct linear_window {
-    const lrow<P>& top;
-    const lrow<P>& mid;
-    const lrow<P>& bot;
+    static constexpr int NS = Targets::size;
+    static constexpr bool row_sums = NS > 0;
+    const lrow<P, NS>& top;
+    const lrow<P, NS>& mid;
+    const lrow<P, NS>& bot;
     word yo;    // all ones when the row is odd
     int t;
 
     __device__ word yodd() const { return yo; }
 
+    template <long long K, int dy>
+    __device__ void rsum(word& s, word& c) const {
+        constexpr int j = kindex<K, Targets>::value;
+        static_assert(j >= 0, "row sums are kept for the rule's counted states only");
+        s = row<dy>().rs_s[j];
+        c = row<dy>().rs_c[j];
+    }
+
     template <int dy>
-    __device__ const lrow<P>& row() const {
+    __device__ const lrow<P, NS>& row() const {
         if constexpr (dy < 0) return top;
         else if constexpr (dy == 0) return mid;
         else return bot;
@@ -186,7 +204,7 @@ struct linear_window {
 
     template <int dx, int dy>
     __device__ vint<P> nb() const {
-        const lrow<P>& r = row<dy>();
+        const lrow<P, NS>& r = row<dy>();
         vint<P> v;
 #pragma unroll
         for (int p = 0; p < P; ++p) v.p[p] = shift1<dx>(r.w[p], r.l[p], r.r[p]);
@@ -196,7 +214,7 @@ struct linear_window {
     // F is per cell, so F of the neighbour's edge half is the edge half of F of the neighbour
     template <int dx, int dy, class F>
     __device__ word nbf(F f) const {
-        const lrow<P>& r = row<dy>();
+        const lrow<P, NS>& r = row<dy>();
         vint<P> own;
 #pragma unroll
         for (int p = 0; p < P; ++p) own.p[p] = r.w[p];
@@ -298,6 +316,34 @@ __device__ __forceinline__ void buf_store(word v, __amdgpu_buffer_rsrc_t r, uint
 // registers instead of moving them; also the prefetch distance in rows.
 constexpr int unroll_rows(int T) { return T <= 2 ? 6 : 3; }
 
+// Row sums of a linear ring slot: for each counted state K (dictionary index), the mask of
+// the slot's row and its two 1-cell shifts, added as a full adder. Called once per row and
+// stage, when the row enters the ring.
+template <class Targets>
+struct row_sum_fill;
+template <long long... K>
+struct row_sum_fill<klist<K...>> {
+    template <int P, int NS>
+    __device__ __forceinline__ static void run(lrow<P, NS>& r) {
+        (one<K, kindex<K, klist<K...>>::value>(r), ...);
+    }
+    template <long long Kj, int j, int P, int NS>
+    __device__ __forceinline__ static void one(lrow<P, NS>& r) {
+        using namespace cellato::evaluators::bit_planes;
+        const is_index<Kj> f{};
+        vint<P> own, side_l, side_r;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            own.p[p] = r.w[p];
+            side_l.p[p] = join(0u, r.l[p]);
+            side_r.p[p] = join(r.r[p], 0u);
+        }
+        const word m = f(own);
+        const uint32_t lh = hi32(f(side_l)), rl = lo32(f(side_r));
+        fa(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl), r.rs_s[j], r.rs_c[j]);
+    }
+};
+
 // ---- one task: a band of columns x a chunk of rows, T generations --------------------------
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
@@ -363,7 +409,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
 
     // generation-k rows, a ring of 3 slots per stage (linear rows carry their neighbours'
     // edge halves; tiled rows exchange whole words when used)
-    lrow<P> lwin[T][3];
+    using targets = typename Eval::row_sum_targets;
+    constexpr int NS = targets::size;
+    lrow<P, NS> lwin[T][3];
     word win[T][3][P];
     word hwin[3][P];       // halo column, generation 0 (HALO_LOAD, tiled)
     word pf[U][P];         // prefetched generation-0 rows
@@ -378,6 +426,15 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 lwin[k][i].w[p] = 0;
                 lwin[k][i].l[p] = 0;
                 lwin[k][i].r[p] = 0;
+            }
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < (NS > 0 ? NS : 1); ++j) {
+                lwin[k][i].rs_s[j] = 0;
+                lwin[k][i].rs_c[j] = 0;
             }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -433,6 +490,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     if constexpr (LOADS_HALO) hwin[nw][p] = hpf[u][p];
                 }
             }
+            if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
             load_row(pf[u], hpf[u]);
 
             // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k
@@ -443,7 +501,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 if constexpr (LAYOUT == LINEAR) {
                     int y = ycur - k;          // only rules with a Margolus block read it
                     while (y < 0) y += a.h;
-                    const linear_window<P> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
+                    const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
                     out = Eval::evaluate(wnd);
                 } else {
                     const tiled_window<P, X, LOADS_HALO> wnd{win[k][tp], win[k][md], win[k][nw],
@@ -462,6 +520,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                             win[kn][nw][p] = out.p[p];
                         }
                     }
+                    if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
                 }
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))

@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--tick-passes", type=int, default=16, help="passes per live job and serving tick")
     ap.add_argument("--serve-baseline-limit", type=float, default=45.0,
                     help="seconds after which the one-at-a-time baseline stops admitting requests")
+    ap.add_argument("--phase", choices=["variants", "batch", "serve", "cpu"], default=None,
+                    help="run one phase in this process (bench.py runs each phase in a child process)")
+    ap.add_argument("--tuner-cache", default=None, help="autotuner measurements shared by the phases")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also time a plain device copy of this many MiB (the calibration kernel for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -128,9 +131,10 @@ def cpu_baseline(C, O):
             "standard_evaluator_1core": {"value": round(std, 4), "sample": "1024^2, 4 generations"}}
 
 
-def make_tuner():
+def make_tuner(cache_path=None):
     """Depth tuner guided by the cost model, trained on the GPU on the committed MI355X
-    dataset; measures the model's two best depths per (automaton, size) bucket."""
+    dataset; measures the model's two best depths per (automaton, size) bucket (measurements
+    kept in `cache_path`, shared by the phases of one bench run)."""
     import numpy as np
     from cellato_amd.autotune import Tuner, features
     from cellato_amd.costmodel import CostModel
@@ -142,7 +146,7 @@ def make_tuner():
     model = CostModel(X.shape[1], hidden=(64, 64), seed=0)
     model.fit(X, y, epochs=600)
     train_s = time.perf_counter() - t
-    return Tuner(model=model, budget=2, target_cells=1 << 26), train_s
+    return Tuner(cache_path=cache_path, model=model, budget=2, target_cells=1 << 26), train_s
 
 
 def bench_batch(C, O, torch, tuner, n):
@@ -272,12 +276,8 @@ def valu_roofline(g, size):
             "instructions_per_launch": ins}
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (each box has one MI355X)",
-              file=sys.stderr)
-        sys.exit(2)
+def phase_variants(a):
+    """The plain-GPU configuration: every (automaton, traverser, depth) variant at a.size^2."""
     import torch
     import cellato_amd as C
     import oracle_lib as O
@@ -309,6 +309,7 @@ def main():
         del dev
         torch.cuda.empty_cache()
 
+    copy_rec = None
     if a.copy_mib:
         n = a.copy_mib * 2**20 // 8
         src = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -358,24 +359,97 @@ def main():
             "unit": "GB/s", "frac": round(simple_gol["algorithmic_GBps"] / HBM_PEAK_GBPS, 4)},
         "variants": variants,
     }
-    if not (a.no_batch and a.no_serve):
-        tuner, train_s = make_tuner()
-        from cellato_amd.tuning_eval import evaluate
-        gains = {h: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in evaluate(budget=2, holdout=h).items()}
-                 for h in ("size", "automaton")}
-        out["tuner"] = {"model": "MLP 15-64-64-1 trained on the GPU (csrc/mlp.hip) on "
-                                 "cellato_amd/data/depth_tuning_mi355x.json", "train_seconds": round(train_s, 2),
-                        "measured_per_bucket": tuner.budget,
-                        "held_out_fraction_of_best": gains}
-        if not a.no_batch:
-            out["batched"] = bench_batch(C, O, torch, tuner, a.batch_grids)
-            print(json.dumps({"batched": out["batched"]}), file=sys.stderr, flush=True)
-        if not a.no_serve:
-            out["served"] = bench_serve(C, O, torch, tuner, a.serve_requests, a.serve_rate, a.serve_baseline_limit,
-                                        a.tick_passes)
-            print(json.dumps({"served": out["served"]}), file=sys.stderr, flush=True)
-    if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(C, O)
+    if copy_rec:
+        out["copy"] = copy_rec
+    return out
+
+
+def phase_batch(a):
+    import torch
+    import cellato_amd as C
+    import oracle_lib as O
+    from cellato_amd.tuning_eval import evaluate
+    tuner, train_s = make_tuner(a.tuner_cache)
+    gains = {h: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in evaluate(budget=2, holdout=h).items()}
+             for h in ("size", "automaton")}
+    out = {"tuner": {"model": "MLP 15-64-64-1 trained on the GPU (csrc/mlp.hip) on "
+                              "cellato_amd/data/depth_tuning_mi355x.json", "train_seconds": round(train_s, 2),
+                     "measured_per_bucket": tuner.budget, "held_out_fraction_of_best": gains}}
+    out["batched"] = bench_batch(C, O, torch, tuner, a.batch_grids)
+    print(json.dumps({"batched": out["batched"]}), file=sys.stderr, flush=True)
+    return out
+
+
+def phase_serve(a):
+    import torch
+    import cellato_amd as C
+    import oracle_lib as O
+    tuner, _ = make_tuner(a.tuner_cache)
+    out = {"served": bench_serve(C, O, torch, tuner, a.serve_requests, a.serve_rate, a.serve_baseline_limit,
+                                 a.tick_passes)}
+    print(json.dumps(out), file=sys.stderr, flush=True)
+    return out
+
+
+def phase_cpu(a):
+    import cellato_amd as C
+    import oracle_lib as O
+    return {"cpu_baseline": cpu_baseline(C, O)}
+
+
+PHASES = {"variants": phase_variants, "batch": phase_batch, "serve": phase_serve, "cpu": phase_cpu}
+
+
+def run_phase(name, argv):
+    """Run one phase in a child process (its own GPU context; a crash there cannot take the
+    whole bench line with it). Returns (result dict or None, error dict or None)."""
+    import collections
+    import subprocess
+    cmd = [sys.executable, "-X", "faulthandler", "-u", str(Path(__file__).resolve()), "--phase", name, *argv]
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    tail = collections.deque(maxlen=40)
+    for line in p.stderr:           # progress lines pass straight through
+        sys.stderr.write(line)
+        sys.stderr.flush()
+        tail.append(line.rstrip())
+    out = p.stdout.read()
+    rc = p.wait()
+    lines = [l for l in out.strip().splitlines() if l.startswith("{")]
+    if rc == 0 and lines:
+        return json.loads(lines[-1]), None
+    return None, {"phase": name, "exit_status": rc, "stderr_tail": list(tail)[-15:]}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (each box has one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.phase:
+        print(json.dumps(PHASES[a.phase](a)), flush=True)
+        return
+    import tempfile
+    argv = [x for x in sys.argv[1:]]
+    with tempfile.TemporaryDirectory() as tmp:
+        argv += ["--tuner-cache", str(Path(tmp) / "tuner_cache.json")]
+        out, err = run_phase("variants", argv)
+        errors = [err] if err else []
+        if out is None:   # no headline: still one JSON line, saying what failed
+            out = {"metric": "GCUPS (cell updates/s) at 16384^2 bit-exact; batched/served vs plain GPU; p99",
+                   "value": None, "unit": "GCUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+                   "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                   "dtype": "u64", "data": "synthetic", "config": {"workload": "game-of-life 16384x16384"}}
+        for name, skip in (("batch", a.no_batch), ("serve", a.no_serve), ("cpu", a.no_cpu)):
+            if skip:
+                continue
+            res, err = run_phase(name, argv)
+            if res:
+                out.update(res)
+            if err:
+                errors.append(err)
+        if errors:
+            out["phase_errors"] = errors
     print(json.dumps(out), flush=True)
 
 

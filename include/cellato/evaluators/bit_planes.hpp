@@ -193,6 +193,101 @@ CELLATO_HD word swap_pairs(word v) {
     return ((v & even) << 1) | ((v >> 1) & even);
 }
 
+// ---- per-row sums of the counted states (streaming callers) --------------------------------
+// A caller that streams rows can work out once per row, for every state the rule counts over
+// the Moore neighbourhood, the 3-cell horizontal sum (cells x-1, x, x+1) of the mask "cell
+// is in that state" as a sum bit s and a carry c, and hand it to the three output rows that
+// read the row (as the row above, its own row, the row below). Such a window declares
+//
+//   static constexpr bool row_sums = true;
+//   template <long long K, int dy> void rsum(word& s, word& c) const   // K = dictionary index
+//
+// and the Moore counts below are formed from those sums: the own row's sum minus the cell
+// itself, plus the sums above and below, gives the count as b0 + 2 (s1 + k + 2 m1).
+// Comparisons of a count with a constant are answered from (b0, k, s1, m1) without forming
+// the count.
+
+template <long long... K>
+struct klist {
+    static constexpr int size = sizeof...(K);
+};
+
+// union of lists, first occurrence order
+template <class A, class B> struct kjoin;
+template <long long... A>
+struct kjoin<klist<A...>, klist<>> { using type = klist<A...>; };
+template <long long... A, long long B0, long long... B>
+struct kjoin<klist<A...>, klist<B0, B...>> {
+    using type = typename kjoin<std::conditional_t<((A == B0) || ...), klist<A...>, klist<A..., B0>>, klist<B...>>::type;
+};
+template <class... L> struct kjoin_all { using type = klist<>; };
+template <class L0, class... L>
+struct kjoin_all<L0, L...> { using type = typename kjoin<L0, typename kjoin_all<L...>::type>::type; };
+
+// position of K in a list (-1 if absent)
+template <long long K, class L> struct kindex;
+template <long long K>
+struct kindex<K, klist<>> { static constexpr int value = -1; };
+template <long long K, long long H, long long... T>
+struct kindex<K, klist<H, T...>> {
+    static constexpr int value = K == H ? 0 : (kindex<K, klist<T...>>::value < 0 ? -1 : 1 + kindex<K, klist<T...>>::value);
+};
+
+template <class W>
+constexpr bool has_row_sums() {
+    if constexpr (requires { W::row_sums; }) return W::row_sums;
+    else return false;
+}
+
+// the Moore count (0..8) of a counted state around a cell, in adder form
+struct moore_sum {
+    word b0, k, s1, m1;   // count = b0 + 2 (s1 + k + 2 m1)
+};
+
+// q = s1 + k + 2 m1 (0..4): q == j, q >= j
+template <int j>
+CELLATO_HD word q_is(const moore_sum& a) {
+    if constexpr (j == 0) return ~(a.s1 | a.k | a.m1);
+    else if constexpr (j == 1) return (a.s1 ^ a.k) & ~a.m1;
+    else if constexpr (j == 2) return (a.s1 & a.k & ~a.m1) | (~(a.s1 | a.k) & a.m1);
+    else if constexpr (j == 3) return (a.s1 ^ a.k) & a.m1;
+    else if constexpr (j == 4) return a.s1 & a.k & a.m1;
+    else return word(0);
+}
+template <int j>
+CELLATO_HD word q_ge(const moore_sum& a) {
+    if constexpr (j <= 0) return ~word(0);
+    else if constexpr (j == 1) return a.s1 | a.k | a.m1;
+    else if constexpr (j == 2) return a.m1 | (a.s1 & a.k);
+    else if constexpr (j == 3) return a.m1 & (a.s1 | a.k);
+    else if constexpr (j == 4) return a.s1 & a.k & a.m1;
+    else return word(0);
+}
+// count == K, count > K
+template <long long K>
+CELLATO_HD word count_is(const moore_sum& a) {
+    if constexpr (K < 0 || K > 8) return word(0);
+    else if constexpr (K % 2) return a.b0 & q_is<static_cast<int>((K - 1) / 2)>(a);
+    else return ~a.b0 & q_is<static_cast<int>(K / 2)>(a);
+}
+template <long long K>
+CELLATO_HD word count_gt(const moore_sum& a) {
+    if constexpr (K < 0) return ~word(0);
+    else if constexpr (K >= 8) return word(0);
+    else if constexpr (K % 2) return q_ge<static_cast<int>((K + 1) / 2)>(a);
+    else return q_ge<static_cast<int>(K / 2 + 1)>(a) | (a.b0 & q_ge<static_cast<int>(K / 2)>(a));
+}
+// the count as a 4-plane value
+CELLATO_HD vint<4> count_value(const moore_sum& a) {
+    const word cy = a.s1 & a.k;
+    vint<4> r;
+    r.p[0] = a.b0;
+    r.p[1] = a.s1 ^ a.k;
+    r.p[2] = a.m1 ^ cy;
+    r.p[3] = a.m1 & cy;
+    return r;
+}
+
 // ---- evaluator ------------------------------------------------------------------------------
 template <class Dict, class E>
 struct ev;
@@ -211,6 +306,31 @@ struct is_index {
     template <int N>
     CELLATO_HD word operator()(const vint<N>& v) const { return eq_const<K>(v); }
 };
+
+// the Moore count of dictionary index K from a window with row sums: the own row's 3-cell sum
+// (sm, cm) less the cell itself (m), then the three rows added column by column
+template <long long K, class Win>
+CELLATO_HD moore_sum moore_sum_of(const Win& w) {
+    word st, ct, sm, cm, sb, cb;
+    w.template rsum<K, -1>(st, ct);
+    w.template rsum<K, 0>(sm, cm);
+    w.template rsum<K, 1>(sb, cb);
+    const word m = w.template nbf<0, 0>(is_index<K>{});
+    const word e0 = sm ^ m, e1 = cm & (sm | ~m);   // (sm + 2 cm) - m, never below 0
+    moore_sum r;
+    fa(st, e0, sb, r.b0, r.k);
+    fa(ct, e1, cb, r.s1, r.m1);
+    return r;
+}
+
+// the states a rule counts over the Moore neighbourhood (dictionary indices): what a
+// streaming caller keeps row sums of
+template <class Dict, class E>
+struct row_targets { using type = klist<>; };
+template <class Dict, template <class...> class N, class... A>
+struct row_targets<Dict, N<A...>> { using type = typename kjoin_all<typename row_targets<Dict, A>::type...>::type; };
+template <class Dict, auto V>
+struct row_targets<Dict, count_neighbors<state_constant<V>, moore_8_neighbors>> { using type = klist<index_of<Dict, V>>; };
 
 template <class Dict, auto V>
 struct ev<Dict, constant<V>> {
@@ -387,9 +507,43 @@ template <class Dict, class C>
 struct ev<Dict, count_neighbors<C, moore_8_neighbors>> {
     template <class Win>
     CELLATO_HD static vint<4> at(const Win& w) {
-        word m[8];
-        moore_masks<Dict, C>::get(w, m);
-        return count8(m);
+        if constexpr (is_state_literal<C>::value && has_row_sums<Win>()) {
+            return count_value(moore_sum_of<index_of<Dict, C::value>>(w));
+        } else {
+            word m[8];
+            moore_masks<Dict, C>::get(w, m);
+            return count8(m);
+        }
+    }
+};
+
+// a Moore count of a state compared with a constant: from the row sums when the window has
+// them, without forming the count
+template <class Dict, auto V, auto K>
+struct ev<Dict, equals<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<K>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        if constexpr (has_row_sums<Win>())
+            return count_is<static_cast<long long>(K)>(moore_sum_of<index_of<Dict, V>>(w));
+        else
+            return eq(ev<Dict, count_neighbors<state_constant<V>, moore_8_neighbors>>::at(w), make_const<static_cast<long long>(K)>());
+    }
+};
+template <class Dict, auto V, auto K>
+struct ev<Dict, not_equals<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<K>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        return ~ev<Dict, equals<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<K>>>::at(w);
+    }
+};
+template <class Dict, auto V, auto K>
+struct ev<Dict, less_than<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<K>>> {
+    template <class Win>
+    CELLATO_HD static word at(const Win& w) {
+        if constexpr (has_row_sums<Win>())
+            return ~count_gt<static_cast<long long>(K) - 1>(moore_sum_of<index_of<Dict, V>>(w));
+        else
+            return gt(make_const<static_cast<long long>(K)>(), ev<Dict, count_neighbors<state_constant<V>, moore_8_neighbors>>::at(w));
     }
 };
 
@@ -408,7 +562,17 @@ template <class Dict, class C, auto Z>
 struct ev<Dict, greater_than<count_neighbors<C, moore_8_neighbors>, constant<Z>>> {
     template <class Win>
     CELLATO_HD static word at(const Win& w) {
-        if constexpr (Z == 0) {
+        if constexpr (is_state_literal<C>::value && has_row_sums<Win>() && Z == 0) {
+            // any neighbour: a row sum above or below is not 0, or the own row holds more than the cell
+            constexpr long long K = index_of<Dict, C::value>;
+            word st, ct, sm, cm, sb, cb;
+            w.template rsum<K, -1>(st, ct);
+            w.template rsum<K, 0>(sm, cm);
+            w.template rsum<K, 1>(sb, cb);
+            return st | ct | sb | cb | cm | (sm & ~w.template nbf<0, 0>(is_index<K>{}));
+        } else if constexpr (is_state_literal<C>::value && has_row_sums<Win>()) {
+            return count_gt<static_cast<long long>(Z)>(moore_sum_of<index_of<Dict, C::value>>(w));
+        } else if constexpr (Z == 0) {
             word m[8];
             moore_masks<Dict, C>::get(w, m);
             return m[0] | m[1] | m[2] | m[3] | m[4] | m[5] | m[6] | m[7];
@@ -548,6 +712,8 @@ struct evaluator {
     using dictionary = state_dictionary_type;
     using expression = Expression;
     static constexpr int planes = state_dictionary_type::needed_bits;
+    // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of
+    using row_sum_targets = typename row_targets<state_dictionary_type, Expression>::type;
 
     template <class Win>
     CELLATO_HD static vint<planes> evaluate(const Win& w) {
