@@ -60,7 +60,10 @@ def parse():
     ap.add_argument("--tick-passes", type=int, default=16, help="passes per live job and serving tick")
     ap.add_argument("--serve-baseline-limit", type=float, default=45.0,
                     help="seconds after which the one-at-a-time baseline stops admitting requests")
-    ap.add_argument("--phase", choices=["variants", "batch", "serve", "cpu"], default=None,
+    ap.add_argument("--no-multiplane", action="store_true", help="skip the multi-plane rules")
+    ap.add_argument("--multiplane", default="cyclic,excitable,fluid,critters")
+    ap.add_argument("--multiplane-steps", type=int, default=400)
+    ap.add_argument("--phase", choices=["variants", "multiplane", "batch", "serve", "cpu"], default=None,
                     help="run one phase in this process (bench.py runs each phase in a child process)")
     ap.add_argument("--tuner-cache", default=None, help="autotuner measurements shared by the phases")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -364,6 +367,43 @@ def phase_variants(a):
     return out
 
 
+# the reference's best published H100 figures (GCUPS) for the multi-plane rules at 16384^2,
+# any traverser (results/H100/_h100.csv: cyclic :2717, excitable :1362, fluid :2290,
+# critters :1664)
+H100_BEST = {"cyclic": 1505, "excitable": 3146, "fluid": 9197, "critters": 4680}
+
+
+def phase_multiplane(a):
+    """Multi-plane rules on the plain GPU path: Cyclic (5 planes), Greenberg-Hastings (3),
+    HPP fluid (4), Critters (1, Margolus), 16384^2, seeded i.i.d., temporal traverser at
+    depths 2/4/8; per automaton the best depth, each re-run bit-exact on a 2048^2 subset."""
+    import torch
+    import cellato_amd as C
+    import oracle_lib as O
+    res = {}
+    for name in a.multiplane.split(","):
+        rule = C.rule(name)
+        dev = C.DeviceGrid(rule, a.size, a.size, 0)
+        planes = C.grid.pack(rule.index_of(C.iid_values(name, a.size, a.size, 2026)), rule.planes, 0)
+        runs = {}
+        for T in (2, 4, 8):
+            dev.upload(planes)
+            dev.cur = 0
+            gcups, ms, ms_launch, _ = time_variant(C, torch, rule, dev, "temporal", T, a.multiplane_steps, 10, 0)
+            runs[T] = round(gcups, 1)
+        bestT = max(runs, key=runs.get)
+        rec = {"planes": rule.planes, "gcups": runs[bestT], "temporal_steps": bestT,
+               "gcups_by_depth": {f"T{T}": g for T, g in runs.items()},
+               "bit_exact_2048_subset": check_subset(C, O, name, "temporal", bestT),
+               "reference_h100_best_gcups": H100_BEST.get(name)}
+        print(json.dumps({name: rec}), file=sys.stderr, flush=True)
+        res[name] = rec
+        del dev
+        torch.cuda.empty_cache()
+    return {"multi_plane": {"workload": f"16384^2, seeded i.i.d., {a.multiplane_steps} generations per depth, "
+                                         "plain GPU path, linear bit planes", "automata": res}}
+
+
 def phase_batch(a):
     import torch
     import cellato_amd as C
@@ -397,7 +437,8 @@ def phase_cpu(a):
     return {"cpu_baseline": cpu_baseline(C, O)}
 
 
-PHASES = {"variants": phase_variants, "batch": phase_batch, "serve": phase_serve, "cpu": phase_cpu}
+PHASES = {"variants": phase_variants, "multiplane": phase_multiplane, "batch": phase_batch, "serve": phase_serve,
+          "cpu": phase_cpu}
 
 
 def run_phase(name, argv):
@@ -440,7 +481,8 @@ def main():
                    "value": None, "unit": "GCUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                    "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                    "dtype": "u64", "data": "synthetic", "config": {"workload": "game-of-life 16384x16384"}}
-        for name, skip in (("batch", a.no_batch), ("serve", a.no_serve), ("cpu", a.no_cpu)):
+        for name, skip in (("multiplane", a.no_multiplane), ("batch", a.no_batch), ("serve", a.no_serve),
+                           ("cpu", a.no_cpu)):
             if skip:
                 continue
             res, err = run_phase(name, argv)
