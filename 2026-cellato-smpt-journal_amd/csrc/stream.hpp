@@ -231,9 +231,19 @@ struct linear_window {
     }
 };
 
+// Row sums of a tiled ring slot: per counted state, the 3-cell horizontal sum of its mask for
+// each cell of the 8x8 block (sum bit, carry), formed once when the block enters the ring.
+template <int NS>
+struct tsums {
+    word s[NS > 0 ? NS : 1];
+    word c[NS > 0 ? NS : 1];
+};
+
 // Tiled layout: a word is an 8x8 block; cell (x, y) at bit (y % 8) * 8 + x % 8.
-template <int P, int X, bool HALO>
+template <int P, int X, bool HALO, class Targets = klist<>>
 struct tiled_window {
+    static constexpr int NS = Targets::size;
+    static constexpr bool row_sums = NS > 0;
     const row_t<P>& top;
     const row_t<P>& mid;
     const row_t<P>& bot;
@@ -242,11 +252,32 @@ struct tiled_window {
     const row_t<P>& hbot;
     xch<X> x;
     int t;
+    const tsums<NS>& stop;
+    const tsums<NS>& smid;
+    const tsums<NS>& sbot;
 
     static constexpr word COL0 = 0x0101010101010101ULL;
     static constexpr word COL7 = 0x8080808080808080ULL;
 
     __device__ word yodd() const { return 0xFF00FF00FF00FF00ULL; }   // rows 1, 3, 5, 7 of a block
+
+    // the block of sums shifted by one cell row (dy = -1: each cell gets the row above)
+    template <int dy>
+    __device__ static word vsum(word t_, word m_, word b_) {
+        if constexpr (dy == 0) return m_;
+        else if constexpr (dy < 0)
+            return join(__builtin_amdgcn_alignbit(lo32(m_), hi32(t_), 24), __builtin_amdgcn_alignbit(hi32(m_), lo32(m_), 24));
+        else
+            return join(__builtin_amdgcn_alignbit(hi32(m_), lo32(m_), 8), __builtin_amdgcn_alignbit(lo32(b_), hi32(m_), 8));
+    }
+
+    template <long long K, int dy>
+    __device__ void rsum(word& s, word& c) const {
+        constexpr int j = kindex<K, Targets>::value;
+        static_assert(j >= 0, "row sums are kept for the rule's counted states only");
+        s = vsum<dy>(stop.s[j], smid.s[j], sbot.s[j]);
+        c = vsum<dy>(stop.c[j], smid.c[j], sbot.c[j]);
+    }
 
     // the block shifted by one cell row, filled from the block above / below
     template <int dy>
@@ -344,6 +375,45 @@ struct row_sum_fill<klist<K...>> {
     }
 };
 
+// Row sums of a tiled ring slot: the counted state's mask of the block and of its left and
+// right neighbour blocks (two lane exchanges of one word per state), shifted by one cell
+// within each 8-cell block row, added as a full adder. `halo` = the block outside the
+// segment end for end lanes that load it (one generation per launch), else unused.
+template <class Targets>
+struct tiled_sum_fill;
+template <long long... K>
+struct tiled_sum_fill<klist<K...>> {
+    template <bool LOADS_HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void run(const word (&w)[P], const word (&halo)[P], const XC& xc, tsums<NS>& out) {
+        (one<K, kindex<K, klist<K...>>::value, LOADS_HALO>(w, halo, xc, out), ...);
+    }
+    template <long long Kj, int j, bool LOADS_HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void one(const word (&w)[P], const word (&halo)[P], const XC& xc, tsums<NS>& out) {
+        using namespace cellato::evaluators::bit_planes;
+        constexpr word C0 = 0x0101010101010101ULL, C7 = 0x8080808080808080ULL;
+        const is_index<Kj> f{};
+        vint<P> own, hv;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            own.p[p] = w[p];
+            hv.p[p] = halo[p];
+        }
+        const word m = f(own);
+        word ml, mr;   // the mask of the block to the left / right
+        if constexpr (LOADS_HALO) {
+            const word mh = f(hv);
+            ml = join(xc.left(lo32(m), lo32(mh)), xc.left(hi32(m), hi32(mh)));
+            mr = join(xc.right(lo32(m), lo32(mh)), xc.right(hi32(m), hi32(mh)));
+        } else {
+            ml = join(xc.left0(lo32(m)), xc.left0(hi32(m)));
+            mr = join(xc.right0(lo32(m)), xc.right0(hi32(m)));
+        }
+        const word at_left = ((m << 1) & ~C0) | ((ml >> 7) & C0);    // cell x - 1
+        const word at_right = ((m >> 1) & ~C7) | ((mr << 7) & C7);   // cell x + 1
+        fa(at_left, m, at_right, out.s[j], out.c[j]);
+    }
+};
+
 // ---- one task: a band of columns x a chunk of rows, T generations --------------------------
 template <class Eval, int LAYOUT, int T, int X, int HALO>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
@@ -414,6 +484,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     lrow<P, NS> lwin[T][3];
     word win[T][3][P];
     word hwin[3][P];       // halo column, generation 0 (HALO_LOAD, tiled)
+    tsums<NS> tsum[T][3];  // row sums of the tiled ring slots
     word pf[U][P];         // prefetched generation-0 rows
     word hpf[U][P];
 #pragma unroll
@@ -440,6 +511,15 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int p = 0; p < P; ++p) hwin[i][p] = 0;
+#pragma unroll
+    for (int k = 0; k < T; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < (NS > 0 ? NS : 1); ++j) {
+                tsum[k][i].s[j] = 0;
+                tsum[k][i].c[j] = 0;
+            }
 
     // row of the next load, kept wrapped onto [0, h) incrementally (no division in the loop)
     int lrow_next = wrap(r0 - T, a.h);
@@ -491,6 +571,8 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 }
             }
             if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
+            if constexpr (LAYOUT == TILED && NS > 0)
+                tiled_sum_fill<targets>::template run<LOADS_HALO>(win[0][nw], hwin[nw], xc, tsum[0][nw]);
             load_row(pf[u], hpf[u]);
 
             // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k
@@ -504,8 +586,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
                     out = Eval::evaluate(wnd);
                 } else {
-                    const tiled_window<P, X, LOADS_HALO> wnd{win[k][tp], win[k][md], win[k][nw],
-                                                             hwin[tp], hwin[md], hwin[nw], xc, t};
+                    const tiled_window<P, X, LOADS_HALO, targets> wnd{win[k][tp], win[k][md], win[k][nw],
+                                                                      hwin[tp], hwin[md], hwin[nw], xc, t,
+                                                                      tsum[k][tp], tsum[k][md], tsum[k][nw]};
                     out = Eval::evaluate(wnd);
                 }
                 if (k + 1 < T) {
@@ -521,6 +604,8 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                         }
                     }
                     if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
+                    if constexpr (LAYOUT == TILED && NS > 0)
+                        tiled_sum_fill<targets>::template run<false>(win[kn][nw], hwin[nw], xc, tsum[kn][nw]);
                 }
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))

@@ -64,10 +64,10 @@ struct host_window {
     }
 };
 
-// The linear window of a streaming caller that keeps per-row 3-cell sums of the counted
-// states (the kernels' ring slots do): sums formed here from the grid, row by row.
-template <int P, class Targets>
-struct host_window_rows : host_window<P, false> {
+// The window of a streaming caller that keeps per-row 3-cell sums of the counted states (the
+// kernels' ring slots do, in both layouts): sums formed here from the grid, row by row.
+template <int P, class Targets, bool Tiled>
+struct host_window_rows : host_window<P, Tiled> {
     static constexpr bool row_sums = true;
     template <long long K, int dy>
     void rsum(u64& s, u64& c) const {
@@ -97,7 +97,7 @@ bool check(int W, int H, int t0, int steps) {
             for (int X = 0; X < wx; ++X) {
                 host_window<P, Tiled> w{a.data(), wx, h, X, Y, t0 + k, Y};
                 vint<P> r;
-                if constexpr (Rows && !Tiled) r = E::evaluate(host_window_rows<P, typename E::row_sum_targets>{w});
+                if constexpr (Rows) r = E::evaluate(host_window_rows<P, typename E::row_sum_targets, Tiled>{w});
                 else r = E::evaluate(w);
                 for (int p = 0; p < P; ++p) b[static_cast<size_t>(p) * wx * h + static_cast<size_t>(Y) * wx + X] = r.p[p];
             }
@@ -125,7 +125,8 @@ int main() {
             ok &= check<Cfg, false>(192, 22, t0, 5);
             ok &= check<Cfg, true>(48, 24, t0, 5);
             ok &= check<Cfg, false, true>(192, 22, t0, 5);
-            n += 3;
+            ok &= check<Cfg, true, true>(48, 24, t0, 5);
+            n += 4;
         }
     }, std::make_integer_sequence<int, cellato::automata::RULE_COUNT>{});
     std::printf("%d cases: %s\n", n, ok ? "ALL OK" : "FAILED");
