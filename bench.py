@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--no-multiplane", action="store_true", help="skip the multi-plane rules")
     ap.add_argument("--multiplane", default="cyclic,excitable,fluid,critters")
     ap.add_argument("--multiplane-steps", type=int, default=400)
-    ap.add_argument("--phase", choices=["variants", "multiplane", "batch", "serve", "cpu"], default=None,
+    ap.add_argument("--phase", choices=["variants", "multiplane", "batch", "serve", "cpu", "table"], default=None,
                     help="run one phase in this process (bench.py runs each phase in a child process)")
     ap.add_argument("--tuner-cache", default=None, help="autotuner measurements shared by the phases")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -437,8 +437,45 @@ def phase_cpu(a):
     return {"cpu_baseline": cpu_baseline(C, O)}
 
 
+def phase_table(a):
+    """Every automaton, both layouts, simple and temporal depths 2/4/6/8 at a.size^2 (seeded
+    i.i.d. states made on the device, 200 generations each): the best GCUPS per automaton and
+    layout. Not part of the default run (`--phase table`)."""
+    import torch
+    import cellato_amd as C
+    from cellato_amd import _lib
+    out = {}
+    stream = torch.cuda.current_stream().cuda_stream
+    for r in C.all_rules():
+        for layout in (0, 1):
+            dev = C.DeviceGrid(r, a.size, a.size, layout)
+            dev.fill_random(7)
+            best = {}
+            for kind, T in (("simple", 1), ("temporal", 2), ("temporal", 4), ("temporal", 6), ("temporal", 8)):
+                def adv(n, t0):
+                    if kind == "simple":
+                        in_b = _lib.simple(r.id, layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), a.size, a.size, t0, n, 0, stream)
+                    else:
+                        in_b = _lib.temporal(r.id, layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), a.size, a.size, t0, n, T, 0, stream)
+                    if in_b:
+                        dev.cur = 1 - dev.cur
+                adv(8, 0)
+                torch.cuda.synchronize()
+                s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s0.record()
+                adv(200, 8)
+                s1.record()
+                torch.cuda.synchronize()
+                best[f"{kind}{T if kind == 'temporal' else ''}"] = round(a.size * a.size * 200 / (s0.elapsed_time(s1) * 1e-3) / 1e9, 1)
+            out.setdefault(r.name, {})["linear" if layout == 0 else "tiled"] = best
+            print(json.dumps({r.name: out[r.name]}), file=sys.stderr, flush=True)
+            del dev
+            torch.cuda.empty_cache()
+    return {"table": out}
+
+
 PHASES = {"variants": phase_variants, "multiplane": phase_multiplane, "batch": phase_batch, "serve": phase_serve,
-          "cpu": phase_cpu}
+          "cpu": phase_cpu, "table": phase_table}
 
 
 def run_phase(name, argv):
