@@ -45,20 +45,30 @@ struct batch {
     void* dev = nullptr;            // pass table + task lists
     pass_args* dev_passes = nullptr;
     batch_task* dev_tasks = nullptr;
-    hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t side[SIDE_STREAMS] = {};
-    hipEvent_t fork = nullptr, join[SIDE_STREAMS] = {};
     long long tasks = 0;
     int launches = 0;
 
     ~batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (auto& s : side) if (s) (void)hipStreamDestroy(s);
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+// Streams and events used only while the schedule is captured. Events recorded during a
+// capture stand for nodes of the captured graph, so they (and the streams) are released
+// as soon as the capture ends, while that graph still exists; nothing of the capture is
+// kept once the graph has been instantiated.
+struct capture_streams {
+    hipStream_t origin = nullptr;
+    hipStream_t side[SIDE_STREAMS] = {};
+    hipEvent_t fork = nullptr, join[SIDE_STREAMS] = {};
+
+    ~capture_streams() {
         for (auto& e : join) if (e) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
-        if (dev) (void)hipFree(dev);
+        for (auto& st : side) if (st) (void)hipStreamDestroy(st);
+        if (origin) (void)hipStreamDestroy(origin);
     }
 };
 
@@ -189,43 +199,50 @@ int build(batch& B, const cellato_batch_grid* grids, int n, int max_waves) {
     if (tbytes) BCHECK(hipMemcpy(B.dev_tasks, tasks.data(), tbytes, hipMemcpyHostToDevice));
 
     // capture the schedule into one graph
-    for (auto& s : B.side) BCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    for (auto& e : B.join) BCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    BCHECK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
-    hipStream_t origin;
-    BCHECK(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
-    BCHECK(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
-    int status = CELLATO_OK;
-    for (auto& round : B.rounds) {
-        if (hipEventRecord(B.fork, origin) != hipSuccess) { status = CELLATO_E_HIP; break; }
-        // launches come largest first; each goes to the stream with the least work so far
-        long long load[SIDE_STREAMS] = {};
-        for (size_t j = 0; j < round.size() && status == CELLATO_OK; ++j) {
-            const launch_rec& L = round[j];
-            const int k = j < SIDE_STREAMS ? static_cast<int>(j)
-                                           : static_cast<int>(std::min_element(load, load + SIDE_STREAMS) - load);
-            load[k] += L.cost;
-            hipStream_t s = B.side[k];
-            if (j < SIDE_STREAMS && hipStreamWaitEvent(s, B.fork, 0) != hipSuccess) { status = CELLATO_E_HIP; break; }
-            batch_args args{B.dev_passes, B.dev_tasks + L.task_begin, L.task_count};
-            void* kargs[] = {&args};
-            if (hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, s) != hipSuccess) status = CELLATO_E_HIP;
-        }
-        for (size_t j = 0; j < std::min(round.size(), static_cast<size_t>(SIDE_STREAMS)) && status == CELLATO_OK; ++j) {
-            if (hipEventRecord(B.join[j], B.side[j]) != hipSuccess || hipStreamWaitEvent(origin, B.join[j], 0) != hipSuccess)
-                status = CELLATO_E_HIP;
-        }
-        if (status != CELLATO_OK) break;
-    }
     hipGraph_t graph = nullptr;
-    const hipError_t end = hipStreamEndCapture(origin, &graph);
-    (void)hipStreamDestroy(origin);
+    int status = CELLATO_OK;
+    hipError_t end = hipSuccess;
+    {
+        capture_streams C;
+        for (auto& st : C.side) BCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (auto& e : C.join) BCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        BCHECK(hipEventCreateWithFlags(&C.fork, hipEventDisableTiming));
+        BCHECK(hipStreamCreateWithFlags(&C.origin, hipStreamNonBlocking));
+        BCHECK(hipStreamBeginCapture(C.origin, hipStreamCaptureModeThreadLocal));
+        for (auto& round : B.rounds) {
+            if (hipEventRecord(C.fork, C.origin) != hipSuccess) { status = CELLATO_E_HIP; break; }
+            // launches come largest first; each goes to the stream with the least work so far
+            long long load[SIDE_STREAMS] = {};
+            for (size_t j = 0; j < round.size() && status == CELLATO_OK; ++j) {
+                const launch_rec& L = round[j];
+                const int k = j < SIDE_STREAMS ? static_cast<int>(j)
+                                               : static_cast<int>(std::min_element(load, load + SIDE_STREAMS) - load);
+                load[k] += L.cost;
+                hipStream_t st = C.side[k];
+                if (j < SIDE_STREAMS && hipStreamWaitEvent(st, C.fork, 0) != hipSuccess) { status = CELLATO_E_HIP; break; }
+                batch_args args{B.dev_passes, B.dev_tasks + L.task_begin, L.task_count};
+                void* kargs[] = {&args};
+                if (hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, st) != hipSuccess)
+                    status = CELLATO_E_HIP;
+            }
+            for (size_t j = 0; j < std::min(round.size(), static_cast<size_t>(SIDE_STREAMS)) && status == CELLATO_OK; ++j) {
+                if (hipEventRecord(C.join[j], C.side[j]) != hipSuccess || hipStreamWaitEvent(C.origin, C.join[j], 0) != hipSuccess)
+                    status = CELLATO_E_HIP;
+            }
+            if (status != CELLATO_OK) break;
+        }
+        end = hipStreamEndCapture(C.origin, &graph);
+    }   // capture streams and events released here, the graph still alive
     if (status != CELLATO_OK || end != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         return bfail(CELLATO_E_HIP, "graph capture of the batch failed");
     }
-    B.graph = graph;
-    BCHECK(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    const hipError_t inst = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);   // the executable graph does not need it
+    if (inst != hipSuccess) {
+        B.exec = nullptr;
+        return bfail(CELLATO_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst));
+    }
     return CELLATO_OK;
 }
 

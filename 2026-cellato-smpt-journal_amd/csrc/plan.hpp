@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "cellato_hip.h"
@@ -45,23 +47,42 @@ inline int device_cus() {
     return cus;
 }
 
+// Blocks of a kernel (BLOCK_THREADS threads, no LDS) one CU holds at once, from the
+// kernel's register use; cached per kernel. 8 when it cannot be asked (no device).
+inline int blocks_per_cu(const void* fn) {
+    static std::mutex mu;
+    static std::map<const void*, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, BLOCK_THREADS, 0) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        n = 8;
+    }
+    cache[fn] = n;
+    return n;
+}
+
 // Rows per task when the caller passes 0. Every wave runs one task, and a CU takes its
 // blocks (4 waves, one per SIMD) at once, so a launch lasts about m x (rows a task walks),
 // m = blocks on the busiest CU; a task walks chunk + 2T rows rounded up to whole unrolled
 // blocks of unroll_rows(T) rows. For m = 3 and m = 2 the smallest chunk (at least max(8, 2T),
 // filling its last unrolled block) whose tasks fit in m blocks per CU is found, and the one
 // with fewer rows per SIMD wins; ties keep m = 3 (more waves hide more latency). One block
-// per CU leaves loads exposed for light rules and is never chosen unless the grid is small.
+// per CU leaves loads exposed for light rules and is chosen only when the kernel's registers
+// let a CU hold fewer blocks (resident = that count) or the grid is small.
 // tasks_of(chunk) = the launch's task count with that chunk.
 template <class TasksOf>
-inline int auto_chunk(TasksOf tasks_of, int T, int rows_per_group) {
+inline int auto_chunk(TasksOf tasks_of, int T, int rows_per_group, int resident = 8) {
     const int U = unroll_rows(T);
     const auto align = [&](long long c) { return (c + 2 * T + U - 1) / U * U - 2 * T; };
     const long long lo = align(2 * T > 8 ? 2 * T : 8);
     const long long hi = rows_per_group > 0 ? rows_per_group : 1;
     const long long cus = device_cus();
     long long best = -1, best_cost = 0;
-    for (int m = 3; m >= 2; --m) {
+    const int m_hi = resident < 3 ? resident : 3, m_lo = resident < 2 ? resident : 2;
+    for (int m = m_hi; m >= m_lo; --m) {
         long long c = lo;
         while (c < hi && (tasks_of(static_cast<int>(c)) + 3) / 4 > m * cus) c += U;
         if (c > hi) c = hi;
@@ -142,7 +163,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         return a.nseg * nch + (a.tail_n > 0 ? (h + static_cast<long long>(a.tail_groups) * c - 1) /
                                                    (static_cast<long long>(a.tail_groups) * c) : 0);
     };
-    a.chunk = chunk > 0 ? chunk : auto_chunk(tasks_of, T, rows_per_group);
+    a.chunk = chunk > 0 ? chunk : auto_chunk(tasks_of, T, rows_per_group, blocks_per_cu(pl.fn));
     a.nchunk = (h + a.groups * a.chunk - 1) / (a.groups * a.chunk);
     const long long main_tasks = static_cast<long long>(a.nseg) * a.nchunk;
     const long long tasks = main_tasks + (a.tail_n > 0 ? (h + a.tail_groups * a.chunk - 1) / (a.tail_groups * a.chunk) : 0);

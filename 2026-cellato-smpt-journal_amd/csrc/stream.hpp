@@ -575,10 +575,16 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 tiled_sum_fill<targets>::template run<LOADS_HALO>(win[0][nw], hwin[nw], xc, tsum[0][nw]);
             load_row(pf[u], hpf[u]);
 
-            // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k
+            // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k. The
+            // stored rows [r0, r1) need stage k from row r0 - (T-1-k) on, i.e. from step 2k + 2;
+            // before that (the pipeline fill) stage k is skipped: what it would write to stage
+            // k+1's ring is read only by stage k+1 steps that are skipped too. Rules of more than
+            // 3 planes keep the stages unconditional: the branches cost them registers (HPP at
+            // T=8 drops to one wave per SIMD) and more time than they save.
             vint<P> out;
 #pragma unroll
             for (int k = 0; k < T; ++k) {
+                if (P <= 3 && ib + u < 2 * k + 2) continue;
                 const int t = a.t0 + k;
                 if constexpr (LAYOUT == LINEAR) {
                     int y = ycur - k;          // only rules with a Margolus block read it

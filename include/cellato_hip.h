@@ -105,7 +105,7 @@ int cellato_batch_create(const cellato_batch_grid* grids, int n, int max_waves, 
 int cellato_batch_launch(void* handle, void* stream);
 int cellato_batch_result_in_b(void* handle, int grid);
 int cellato_batch_stats(void* handle, int* rounds, int* launches, long long* tasks);
-int cellato_batch_destroy(void* handle);
+int cellato_batch_destroy(void* handle);   /* after the stream has finished the batch's launches */
 /* Schedule of a batch without building it (host only, no GPU work; the plane pointers must
  * be non-null but are not dereferenced): the round in which each grid's chain of passes starts (start_round, n entries, may
  * be NULL), the round count, kernel launches and tasks. */
