@@ -75,10 +75,18 @@ CELLATO_HD vint<1> as_vint(word m) { vint<1> r; r.p[0] = m; return r; }
 template <int N> CELLATO_HD vint<N> as_vint(const vint<N>& v) { return v; }
 
 // Three-input boolean function `lut` (truth table indexed by a*4 + b*2 + c) per bit; one
-// v_bitop3_b32 per 32-bit half on gfx950.
+// v_bitop3_b32 per 32-bit half on gfx950. With a compile-time constant input the plain
+// expression is kept, so that the compiler can fold it.
 template <unsigned lut>
 CELLATO_HD word lop3(word a, word b, word c) {
 #if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_constant_p(a) || __builtin_constant_p(b) || __builtin_constant_p(c)) {
+        word r = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if ((lut >> i) & 1) r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+        return r;
+    }
     const unsigned lo = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(a), static_cast<unsigned>(b), static_cast<unsigned>(c), lut);
     const unsigned hi = __builtin_amdgcn_bitop3_b32(static_cast<unsigned>(a >> 32), static_cast<unsigned>(b >> 32), static_cast<unsigned>(c >> 32), lut);
     return (word(hi) << 32) | lo;
@@ -109,22 +117,41 @@ CELLATO_HD vint<cmax(A, B) + 1> add(const vint<A>& a, const vint<B>& b) {
     return r;
 }
 
+// a == b per cell: every plane agrees; from the second plane on, one three-input op per plane
+// (r & ~(x ^ y))
 template <int A, int B>
 CELLATO_HD word eq(const vint<A>& a, const vint<B>& b) {
-    word r = ~word(0);
+    word r = ~(plane(a, 0) ^ plane(b, 0));
 #pragma unroll
-    for (int i = 0; i < cmax(A, B); ++i) r &= ~(plane(a, i) ^ plane(b, i));
+    for (int i = 1; i < cmax(A, B); ++i) r = lop3<0x90>(r, plane(a, i), plane(b, i));
     return r;
 }
 
+// a == K per cell: the AND of the planes taken as they are or negated, three literals per
+// three-input op (the truth table of the op has the one row that matches K set)
 template <long long K, int A>
 CELLATO_HD word eq_const(const vint<A>& a) {
     if constexpr (bit_width(K) > A) {
         return word(0);
+    } else if constexpr (A == 1) {
+        return (K & 1) ? a.p[0] : ~a.p[0];
+    } else if constexpr (A == 2) {
+        return ((K & 1) ? a.p[0] : ~a.p[0]) & ((K & 2) ? a.p[1] : ~a.p[1]);
     } else {
-        word r = ~word(0);
+        constexpr unsigned row = 4 * (K & 1) + 2 * ((K >> 1) & 1) + ((K >> 2) & 1);
+        word r = lop3<1u << row>(a.p[0], a.p[1], a.p[2]);
 #pragma unroll
-        for (int i = 0; i < A; ++i) r &= ((K >> i) & 1) ? a.p[i] : ~a.p[i];
+        for (int i = 3; i < A; i += 2) {
+            const bool k0 = (K >> i) & 1, k1 = (K >> (i + 1)) & 1;
+            if (i + 1 < A) {
+                if (k0 && k1) r = lop3<0x80>(r, a.p[i], a.p[i + 1]);
+                else if (k0) r = lop3<0x40>(r, a.p[i], a.p[i + 1]);
+                else if (k1) r = lop3<0x20>(r, a.p[i], a.p[i + 1]);
+                else r = lop3<0x10>(r, a.p[i], a.p[i + 1]);
+            } else {
+                r &= k0 ? a.p[i] : ~a.p[i];
+            }
+        }
         return r;
     }
 }
@@ -156,6 +183,15 @@ CELLATO_HD vint<cmax(A, B)> select(word m, const vint<A>& a, const vint<B>& b) {
     vint<cmax(A, B)> r;
 #pragma unroll
     for (int i = 0; i < cmax(A, B); ++i) r.p[i] = (plane(a, i) & m) | (plane(b, i) & ~m);
+    return r;
+}
+// the same as one three-input op per plane (a rule's if-then-else of computed values: the
+// compiler does not always find that form by itself)
+template <int A, int B>
+CELLATO_HD vint<cmax(A, B)> select_lut(word m, const vint<A>& a, const vint<B>& b) {
+    vint<cmax(A, B)> r;
+#pragma unroll
+    for (int i = 0; i < cmax(A, B); ++i) r.p[i] = lop3<0xCA>(m, plane(a, i), plane(b, i));
     return r;
 }
 CELLATO_HD word select(word m, word a, word b) { return (a & m) | (b & ~m); }
@@ -244,23 +280,24 @@ struct moore_sum {
     word b0, k, s1, m1;   // count = b0 + 2 (s1 + k + 2 m1)
 };
 
-// q = s1 + k + 2 m1 (0..4): q == j, q >= j
+// q = s1 + k + 2 m1 (0..4): q == j, q >= j. Each is one three-input function of (s1, k, m1),
+// written as its truth table (index 4 s1 + 2 k + m1) so that it stays one bitop3 per half.
 template <int j>
 CELLATO_HD word q_is(const moore_sum& a) {
-    if constexpr (j == 0) return ~(a.s1 | a.k | a.m1);
-    else if constexpr (j == 1) return (a.s1 ^ a.k) & ~a.m1;
-    else if constexpr (j == 2) return (a.s1 & a.k & ~a.m1) | (~(a.s1 | a.k) & a.m1);
-    else if constexpr (j == 3) return (a.s1 ^ a.k) & a.m1;
-    else if constexpr (j == 4) return a.s1 & a.k & a.m1;
+    if constexpr (j == 0) return lop3<0x01>(a.s1, a.k, a.m1);        // (0,0,0)
+    else if constexpr (j == 1) return lop3<0x14>(a.s1, a.k, a.m1);   // (1,0,0) (0,1,0)
+    else if constexpr (j == 2) return lop3<0x42>(a.s1, a.k, a.m1);   // (1,1,0) (0,0,1)
+    else if constexpr (j == 3) return lop3<0x28>(a.s1, a.k, a.m1);   // (1,0,1) (0,1,1)
+    else if constexpr (j == 4) return lop3<0x80>(a.s1, a.k, a.m1);   // (1,1,1)
     else return word(0);
 }
 template <int j>
 CELLATO_HD word q_ge(const moore_sum& a) {
     if constexpr (j <= 0) return ~word(0);
-    else if constexpr (j == 1) return a.s1 | a.k | a.m1;
-    else if constexpr (j == 2) return a.m1 | (a.s1 & a.k);
-    else if constexpr (j == 3) return a.m1 & (a.s1 | a.k);
-    else if constexpr (j == 4) return a.s1 & a.k & a.m1;
+    else if constexpr (j == 1) return lop3<0xFE>(a.s1, a.k, a.m1);   // s1 | k | m1
+    else if constexpr (j == 2) return lop3<0xEA>(a.s1, a.k, a.m1);   // m1 | (s1 & k)
+    else if constexpr (j == 3) return lop3<0xA8>(a.s1, a.k, a.m1);   // m1 & (s1 | k)
+    else if constexpr (j == 4) return lop3<0x80>(a.s1, a.k, a.m1);
     else return word(0);
 }
 // count == K, count > K
@@ -316,7 +353,8 @@ CELLATO_HD moore_sum moore_sum_of(const Win& w) {
     w.template rsum<K, 0>(sm, cm);
     w.template rsum<K, 1>(sb, cb);
     const word m = w.template nbf<0, 0>(is_index<K>{});
-    const word e0 = sm ^ m, e1 = cm & (sm | ~m);   // (sm + 2 cm) - m, never below 0
+    // (sm + 2 cm) - m, never below 0: e0 = sm ^ m, e1 = cm & (sm | ~m)
+    const word e0 = sm ^ m, e1 = lop3<0xD0>(cm, sm, m);
     moore_sum r;
     fa(st, e0, sb, r.b0, r.k);
     fa(ct, e1, cb, r.s1, r.m1);
@@ -371,7 +409,7 @@ struct ev<Dict, if_then_else<C, A, B>> {
         if constexpr (std::is_same_v<decltype(a), word> && std::is_same_v<decltype(b), word>)
             return select(c, a, b);
         else
-            return select(c, as_vint(a), as_vint(b));
+            return select_lut(c, as_vint(a), as_vint(b));
     }
 };
 
