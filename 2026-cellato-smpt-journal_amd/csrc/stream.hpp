@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 #include "cellato/evaluators/bit_planes.hpp"
 
@@ -547,7 +548,10 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     for (int u = 0; u < U; ++u) load_row(pf[u], hpf[u]);
     int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
 
-    for (int ib = 0; ib < nsteps; ib += U) {
+    // One block of U steps. FILL: the block may fall in the pipeline fill (steps below 2T),
+    // where stages that no stored row needs are skipped.
+    auto block = [&](const int ib, auto fill_tag) {
+        constexpr bool FILL = decltype(fill_tag)::value;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int s = r0 - T + ib + u;
@@ -578,13 +582,11 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k. The
             // stored rows [r0, r1) need stage k from row r0 - (T-1-k) on, i.e. from step 2k + 2;
             // before that (the pipeline fill) stage k is skipped: what it would write to stage
-            // k+1's ring is read only by stage k+1 steps that are skipped too. Rules of more than
-            // 3 planes keep the stages unconditional: the branches cost them registers (HPP at
-            // T=8 drops to one wave per SIMD) and more time than they save.
+            // k+1's ring is read only by stage k+1 steps that are skipped too.
             vint<P> out;
 #pragma unroll
             for (int k = 0; k < T; ++k) {
-                if (P <= 3 && ib + u < 2 * k + 2) continue;
+                if (FILL && ib + u < 2 * k + 2) continue;
                 const int t = a.t0 + k;
                 if constexpr (LAYOUT == LINEAR) {
                     int y = ycur - k;          // only rules with a Margolus block read it
@@ -625,7 +627,15 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 }
             }
         }
-    }
+    };
+    // The fill runs in a copy of the loop that tests each stage, the steady state in one
+    // without the tests (the tests cost scalar issue slots and keep skipped ring slots alive
+    // as extra registers). Rules of more than 3 planes run every stage from the start: for
+    // them the second copy of the body costs more than the skipped stages save.
+    int ib = 0;
+    if constexpr (P <= 3)
+        for (; ib < 2 * T && ib < nsteps; ib += U) block(ib, std::true_type{});
+    for (; ib < nsteps; ib += U) block(ib, std::false_type{});
 }
 
 // Blocks are dealt round-robin to the 8 XCDs; renumber them so each XCD (own L2) gets a
