@@ -332,6 +332,11 @@ __device__ __forceinline__ int wrap(int v, int m) {
 // column offset a per-lane constant, so a load or store costs no address arithmetic; an
 // offset past the plane reads 0 instead of faulting. Planes are < 4 GiB (checked on the host).
 constexpr int BUFFER_DWORD3 = 0x00020000;   // raw, untyped 32-bit-element access on gfx9
+// An offset past every plane (planes hold at most 2^32 - 8 bytes): a load there reads 0 and a
+// store there is dropped. Lanes that have nothing to load or store issue the instruction with
+// it instead of branching around it, so that the compiler's memory wait counts stay exact
+// (a load or store under a branch makes it wait for the whole prefetch queue).
+constexpr uint32_t OOB_OFFSET = 0xFFFFFFF8u;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const word* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<word*>(p), static_cast<short>(0), static_cast<int>(bytes), BUFFER_DWORD3);
@@ -345,8 +350,13 @@ __device__ __forceinline__ void buf_store(word v, __amdgpu_buffer_rsrc_t r, uint
 }
 
 // Rows per unrolled loop body: a multiple of 3 so that the 3-row windows rotate by renaming
-// registers instead of moving them; also the prefetch distance in rows.
+// registers instead of moving them.
 constexpr int unroll_rows(int T) { return T <= 2 ? 6 : 3; }
+// Rows loaded ahead of use. At the memory-bound depths a loaded row then lives in one register
+// for exactly unroll_rows steps (in flight, then in the 3-row window), so registers rotate by
+// renaming with no copies; a copy would make the wait for its load come early. (Measured at
+// 16384^2: 6 rows per body, 3 ahead beats 9 per body, 6 ahead, for every rule but Critters.)
+constexpr int prefetch_rows(int T) { return T <= 2 ? unroll_rows(T) - 3 : unroll_rows(T); }
 
 // Row sums of a linear ring slot: for each counted state K (dictionary index), the mask of
 // the slot's row and its two 1-cell shifts, added as a full adder. Called once per row and
@@ -423,6 +433,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     // lane groups walk different rows: rows narrower than a wave, and the halo-lane tail
     constexpr bool GROUPED = X == XCH_PERMUTE || HALO == HALO_LANES;
     constexpr int U = unroll_rows(T);
+    constexpr int D = prefetch_rows(T);
     static_assert(!(HALO == HALO_LOAD && T != 1), "loaded halos are exact for one generation only");
 
     const int lane = lane_id();
@@ -533,10 +544,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             hdst[p] = 0;
         }
         if constexpr (LOADS_HALO) {
-            if (end_lane) {
+            const uint32_t hoff = end_lane ? roff + halo_off : OOB_OFFSET;
 #pragma unroll
-                for (int p = 0; p < P; ++p) hdst[p] = buf_load(rin[p], halo_off, roff);
-            }
+            for (int p = 0; p < P; ++p) hdst[p] = buf_load(rin[p], hoff, 0u);
         }
         lrow_next = lrow_next + 1 == a.h ? 0 : lrow_next + 1;
     };
@@ -545,7 +555,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     // loop is uniform across the wave even when groups walk different rows.
     const int nsteps = a.chunk + 2 * T;
 #pragma unroll
-    for (int u = 0; u < U; ++u) load_row(pf[u], hpf[u]);
+    for (int u = 0; u < D; ++u) load_row(pf[u], hpf[u]);
     int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
 
     // One block of U steps. FILL: the block may fall in the pipeline fill (steps below 2T),
@@ -577,7 +587,10 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
             if constexpr (LAYOUT == TILED && NS > 0)
                 tiled_sum_fill<targets>::template run<LOADS_HALO>(win[0][nw], hwin[nw], xc, tsum[0][nw]);
-            load_row(pf[u], hpf[u]);
+            // the load stays after this step's use of the window (hoisted above it, it would
+            // overlap the register it refills with a live row and force a copy)
+            if constexpr (T <= 2) __builtin_amdgcn_sched_barrier(0);
+            load_row(pf[(u + D) % U], hpf[(u + D) % U]);   // the row of step ib + u + D
 
             // stage k: generation k+1 of row s-1-k from generation-k rows s-2-k .. s-k. The
             // stored rows [r0, r1) need stage k from row r0 - (T-1-k) on, i.e. from step 2k + 2;
@@ -618,14 +631,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))
             const int yo = s - T;
-            if (yo >= r0 && yo < r1 && owner) {
-                const uint32_t roff = static_cast<uint32_t>(yo) * row_bytes;
+            const uint32_t soff = yo >= r0 && yo < r1 && owner ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    if constexpr (GROUPED) buf_store(out.p[p], rout[p], roff + col_off, 0u);
-                    else buf_store(out.p[p], rout[p], col_off, roff);
-                }
-            }
+            for (int p = 0; p < P; ++p) buf_store(out.p[p], rout[p], soff, 0u);
         }
     };
     // The fill runs in a copy of the loop that tests each stage, the steady state in one
