@@ -426,7 +426,10 @@ struct tiled_sum_fill<klist<K...>> {
 };
 
 // ---- one task: a band of columns x a chunk of rows, T generations --------------------------
-template <class Eval, int LAYOUT, int T, int X, int HALO>
+// TPAR: -1, or the parity of a.t0 known at compile time; the windows then carry only the
+// parity of each stage's step, and a rule that reads it (alternation, Margolus blocks) keeps
+// one of its two forms.
+template <class Eval, int LAYOUT, int T, int X, int HALO, int TPAR = -1>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     constexpr int P = Eval::planes;
     constexpr bool LOADS_HALO = HALO == HALO_LOAD;
@@ -600,7 +603,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
 #pragma unroll
             for (int k = 0; k < T; ++k) {
                 if (FILL && ib + u < 2 * k + 2) continue;
-                const int t = a.t0 + k;
+                const int t = TPAR < 0 ? a.t0 + k : (TPAR + k) & 1;
                 if constexpr (LAYOUT == LINEAR) {
                     int y = ycur - k;          // only rules with a Margolus block read it
                     while (y < 0) y += a.h;
@@ -657,7 +660,7 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 // One task of a pass whose rows are cut into 62-column halo-lane segments plus a tail of
 // narrow lane groups. Both kinds run the same loop (one copy of the code): only the lane
 // mapping differs, and it is set up here, uniformly across the wave.
-template <class Eval, int LAYOUT, int T>
+template <class Eval, int LAYOUT, int T, int TPAR = -1>
 __device__ __forceinline__ void segmented_task(const pass_args& a, int task) {
     pass_args b = a;
     int t = task;
@@ -669,7 +672,7 @@ __device__ __forceinline__ void segmented_task(const pass_args& a, int task) {
         b.nseg = 1;
         t = task - a.main_tasks;
     }
-    stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES>(b, t);
+    stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES, TPAR>(b, t);
 }
 
 template <class Eval, int LAYOUT, int T, int X, int HALO>
@@ -677,10 +680,21 @@ __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
     const int blk = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int task = __builtin_amdgcn_readfirstlane(blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
     if (task >= a.ntasks) return;
-    if constexpr (HALO == HALO_LANES)
+    if constexpr (Eval::reads_step_parity) {
+        // one copy of the task per parity of the first step (a uniform branch: a launch
+        // runs one of them)
+        if (a.t0 & 1) {
+            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 1>(a, task);
+            else stream_task<Eval, LAYOUT, T, X, HALO, 1>(a, task);
+        } else {
+            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 0>(a, task);
+            else stream_task<Eval, LAYOUT, T, X, HALO, 0>(a, task);
+        }
+    } else if constexpr (HALO == HALO_LANES) {
         segmented_task<Eval, LAYOUT, T>(a, task);
-    else
+    } else {
         stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
+    }
 }
 
 // ---- batches: many grids, one persistent launch per (rule, T) and round ---------------------
@@ -695,6 +709,16 @@ struct batch_args {
     int ntasks;
 };
 
+template <class Eval, int LAYOUT, int T, int TPAR>
+__device__ __forceinline__ void batch_task_run(const pass_args& a, int task) {
+    if (a.variant == VARIANT_SEGMENT)
+        segmented_task<Eval, LAYOUT, T, TPAR>(a, task);
+    else if (a.variant == VARIANT_RING64)
+        stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE, TPAR>(a, task);
+    else
+        stream_task<Eval, LAYOUT, T, XCH_PERMUTE, HALO_NONE, TPAR>(a, task);
+}
+
 // A fixed grid of waves walks a task list that mixes passes of different grids (sizes, lane
 // mappings, start steps); each task runs exactly as in the single-grid kernel.
 template <class Eval, int LAYOUT, int T>
@@ -705,12 +729,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) batch_kernel(batch_args b) {
     for (int i = first; i < b.ntasks; i += nwaves) {
         const batch_task t = b.tasks[i];
         const pass_args& a = b.passes[t.pass];
-        if (a.variant == VARIANT_SEGMENT)
-            segmented_task<Eval, LAYOUT, T>(a, t.task);
-        else if (a.variant == VARIANT_RING64)
-            stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE>(a, t.task);
-        else
-            stream_task<Eval, LAYOUT, T, XCH_PERMUTE, HALO_NONE>(a, t.task);
+        if constexpr (Eval::reads_step_parity) {
+            // passes of even depth that start at step 0 all start on even steps: only one of
+            // the two copies runs
+            if (a.t0 & 1) batch_task_run<Eval, LAYOUT, T, 1>(a, t.task);
+            else batch_task_run<Eval, LAYOUT, T, 0>(a, t.task);
+        } else {
+            batch_task_run<Eval, LAYOUT, T, -1>(a, t.task);
+        }
     }
 }
 

@@ -752,6 +752,10 @@ struct evaluator {
     static constexpr int planes = state_dictionary_type::needed_bits;
     // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of
     using row_sum_targets = typename row_targets<state_dictionary_type, Expression>::type;
+    // the rule reads the parity of the time step (alternating algorithms, Margolus blocks):
+    // a streaming caller may then pass a window whose `t` is only that parity
+    static constexpr bool reads_step_parity =
+        (rule_traits<Expression>::value & (TRAIT_ALTERNATING | TRAIT_MARGOLUS)) != 0;
 
     template <class Win>
     CELLATO_HD static vint<planes> evaluate(const Win& w) {
