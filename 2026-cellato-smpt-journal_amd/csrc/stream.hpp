@@ -641,11 +641,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     };
     // The fill runs in a copy of the loop that tests each stage, the steady state in one
     // without the tests (the tests cost scalar issue slots and keep skipped ring slots alive
-    // as extra registers). Rules of more than 3 planes run every stage from the start: for
-    // them the second copy of the body costs more than the skipped stages save.
+    // as extra registers).
     int ib = 0;
-    if constexpr (P <= 3)
-        for (; ib < 2 * T && ib < nsteps; ib += U) block(ib, std::true_type{});
+    for (; ib < 2 * T && ib < nsteps; ib += U) block(ib, std::true_type{});
     for (; ib < nsteps; ib += U) block(ib, std::false_type{});
 }
 
