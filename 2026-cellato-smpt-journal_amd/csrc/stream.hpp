@@ -240,6 +240,19 @@ struct tsums {
     word c[NS > 0 ? NS : 1];
 };
 
+// The cell to the left (right) of every cell of a tiled word (8 cells per byte row): the byte
+// rows shifted by one bit, the edge column from the block to the left (right). Per 32-bit
+// half: two shifts and one three-input select (truth table 0xE4 = c ? a : b) with the column
+// mask, where the compiler made 64-bit shifts and four mask ops of the same expression.
+__device__ __forceinline__ word tiled_left(word own, word left) {
+    return join(__builtin_amdgcn_bitop3_b32(lo32(own) << 1, lo32(left) >> 7, 0xFEFEFEFEu, 0xE4),
+                __builtin_amdgcn_bitop3_b32(hi32(own) << 1, hi32(left) >> 7, 0xFEFEFEFEu, 0xE4));
+}
+__device__ __forceinline__ word tiled_right(word own, word right) {
+    return join(__builtin_amdgcn_bitop3_b32(lo32(own) >> 1, lo32(right) << 7, 0x7F7F7F7Fu, 0xE4),
+                __builtin_amdgcn_bitop3_b32(hi32(own) >> 1, hi32(right) << 7, 0x7F7F7F7Fu, 0xE4));
+}
+
 // Tiled layout: a word is an 8x8 block; cell (x, y) at bit (y % 8) * 8 + x % 8.
 template <int P, int X, bool HALO, class Targets = klist<>>
 struct tiled_window {
@@ -256,9 +269,6 @@ struct tiled_window {
     const tsums<NS>& stop;
     const tsums<NS>& smid;
     const tsums<NS>& sbot;
-
-    static constexpr word COL0 = 0x0101010101010101ULL;
-    static constexpr word COL7 = 0x8080808080808080ULL;
 
     __device__ word yodd() const { return 0xFF00FF00FF00FF00ULL; }   // rows 1, 3, 5, 7 of a block
 
@@ -294,10 +304,10 @@ struct tiled_window {
             return own;
         } else if constexpr (dx < 0) {
             const word l = join(x.left(lo32(own), HALO ? lo32(halo) : 0u), x.left(hi32(own), HALO ? hi32(halo) : 0u));
-            return ((own << 1) & ~COL0) | ((l >> 7) & COL0);
+            return tiled_left(own, l);
         } else {
             const word r = join(x.right(lo32(own), HALO ? lo32(halo) : 0u), x.right(hi32(own), HALO ? hi32(halo) : 0u));
-            return ((own >> 1) & ~COL7) | ((r << 7) & COL7);
+            return tiled_right(own, r);
         }
     }
 
@@ -401,7 +411,6 @@ struct tiled_sum_fill<klist<K...>> {
     template <long long Kj, int j, bool LOADS_HALO, int P, int NS, class XC>
     __device__ __forceinline__ static void one(const word (&w)[P], const word (&halo)[P], const XC& xc, tsums<NS>& out) {
         using namespace cellato::evaluators::bit_planes;
-        constexpr word C0 = 0x0101010101010101ULL, C7 = 0x8080808080808080ULL;
         const is_index<Kj> f{};
         vint<P> own, hv;
 #pragma unroll
@@ -419,8 +428,8 @@ struct tiled_sum_fill<klist<K...>> {
             ml = join(xc.left0(lo32(m)), xc.left0(hi32(m)));
             mr = join(xc.right0(lo32(m)), xc.right0(hi32(m)));
         }
-        const word at_left = ((m << 1) & ~C0) | ((ml >> 7) & C0);    // cell x - 1
-        const word at_right = ((m >> 1) & ~C7) | ((mr << 7) & C7);   // cell x + 1
+        const word at_left = tiled_left(m, ml);     // cell x - 1
+        const word at_right = tiled_right(m, mr);   // cell x + 1
         fa(at_left, m, at_right, out.s[j], out.c[j]);
     }
 };
