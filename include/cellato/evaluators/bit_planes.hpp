@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "../core/ast.hpp"
 
@@ -742,6 +743,123 @@ struct rule_traits<alternate_algorithms<A...>> {
     static constexpr unsigned value = TRAIT_ALTERNATING | (0u | ... | rule_traits<A>::value);
 };
 
+// ---- rules of the cell and one Moore count (1 plane) -----------------------------------------
+// A 1-plane rule built only from the cell itself, one state's Moore count, constants,
+// comparisons, and / or / not and if-then-else (Game of Life, Maze) is a table T[cell][count].
+// With the count in adder form (count = b0 + 2q, q = s1 + k + 2 m1 in 0..4) the rule becomes,
+// for each 2-input function g of (cell, b0) that T takes at some values of q,
+// [q in S_g] & g(cell, b0), OR-ed over the distinct g: one three-input op for the set test and
+// one for the term. Game of Life: q == 1 & (cell | b0), 2 ops per half word against the 6 the
+// node-by-node evaluation takes.
+template <class Dict, class E>
+struct cc_expr {   // not of that form
+    static constexpr bool ok = false;
+    static constexpr long long K = -1;
+    static constexpr long long at(long long, int) { return 0; }
+};
+template <class Dict, auto V>
+struct cc_expr<Dict, constant<V>> {
+    static constexpr bool ok = true;
+    static constexpr long long K = -1;
+    static constexpr long long at(long long, int) { return static_cast<long long>(V); }
+};
+template <class Dict, auto V>
+struct cc_expr<Dict, state_constant<V>> {
+    static constexpr bool ok = true;
+    static constexpr long long K = -1;
+    static constexpr long long at(long long, int) { return index_of<Dict, V>; }
+};
+template <class Dict>
+struct cc_expr<Dict, neighbor_at<0, 0>> {   // the cell's dictionary index
+    static constexpr bool ok = true;
+    static constexpr long long K = -1;
+    static constexpr long long at(long long cell, int) { return cell; }
+};
+template <class Dict, auto V>
+struct cc_expr<Dict, count_neighbors<state_constant<V>, moore_8_neighbors>> {
+    static constexpr bool ok = true;
+    static constexpr long long K = index_of<Dict, V>;
+    static constexpr long long at(long long, int count) { return count; }
+};
+// the operands' count, if any, must be of one state
+template <class Dict, class... A>
+struct cc_nodes {
+    static constexpr long long K = [] {
+        long long k = -1;
+        ((k = cc_expr<Dict, A>::K >= 0 ? cc_expr<Dict, A>::K : k), ...);
+        return k;
+    }();
+    static constexpr bool ok = (cc_expr<Dict, A>::ok && ...) && ((cc_expr<Dict, A>::K < 0 || cc_expr<Dict, A>::K == K) && ...);
+};
+#define CELLATO_CC_BINARY(node, op)                                                             \
+    template <class Dict, class L, class R>                                                     \
+    struct cc_expr<Dict, node<L, R>> : cc_nodes<Dict, L, R> {                                   \
+        static constexpr long long at(long long c, int n) {                                     \
+            return (cc_expr<Dict, L>::at(c, n) op cc_expr<Dict, R>::at(c, n)) ? 1 : 0;          \
+        }                                                                                       \
+    };
+CELLATO_CC_BINARY(equals, ==)
+CELLATO_CC_BINARY(not_equals, !=)
+CELLATO_CC_BINARY(less_than, <)
+CELLATO_CC_BINARY(greater_than, >)
+CELLATO_CC_BINARY(and_, &&)
+CELLATO_CC_BINARY(or_, ||)
+#undef CELLATO_CC_BINARY
+template <class Dict, class V>
+struct cc_expr<Dict, not_<V>> : cc_nodes<Dict, V> {
+    static constexpr long long at(long long c, int n) { return cc_expr<Dict, V>::at(c, n) ? 0 : 1; }
+};
+template <class Dict, class C, class A, class B>
+struct cc_expr<Dict, if_then_else<C, A, B>> : cc_nodes<Dict, C, A, B> {
+    static constexpr long long at(long long c, int n) {
+        return cc_expr<Dict, C>::at(c, n) ? cc_expr<Dict, A>::at(c, n) : cc_expr<Dict, B>::at(c, n);
+    }
+};
+
+template <class Dict, class E>
+struct cc_rule {
+    using X = cc_expr<Dict, E>;
+    static constexpr bool ok = X::ok && X::K >= 0;
+    // g_q as a truth table over (cell, b0), bit 2 cell + b0 (the result's plane 0). q = 4
+    // means count 8 (b0 is then 0): its b0 = 1 column repeats count 8.
+    static constexpr unsigned g(int q) {
+        unsigned r = 0;
+        for (int c = 0; c < 2; ++c)
+            for (int b = 0; b < 2; ++b) {
+                const int n = 2 * q + b <= 8 ? 2 * q + b : 2 * q;
+                if (X::at(c, n) & 1) r |= 1u << (2 * c + b);
+            }
+        return r;
+    }
+    // truth table over (s1, k, m1) (index 4 s1 + 2 k + m1) of "g_q == G"
+    static constexpr unsigned qset(unsigned G) {
+        unsigned lut = 0;
+        for (int i = 0; i < 8; ++i)
+            if (g(((i >> 2) & 1) + ((i >> 1) & 1) + 2 * (i & 1)) == G) lut |= 1u << i;
+        return lut;
+    }
+};
+
+template <class Dict, class E, unsigned G>
+CELLATO_HD word cc_term(const moore_sum& ms, word cell) {
+    using R = cc_rule<Dict, E>;
+    if constexpr (G == 0 || R::qset(G) == 0) {
+        return word(0);
+    } else {
+        // (q in S_G, cell, b0) -> q in S_G and G(cell, b0): rows 4 + 2 cell + b0
+        return lop3<(G << 4)>(lop3<R::qset(G)>(ms.s1, ms.k, ms.m1), cell, ms.b0);
+    }
+}
+
+template <class Dict, class E, class Win>
+CELLATO_HD word cc_eval(const Win& w) {
+    const moore_sum ms = moore_sum_of<cc_rule<Dict, E>::X::K>(w);
+    const word cell = w.template nbf<0, 0>(is_index<1>{});
+    return [&]<unsigned... G>(std::integer_sequence<unsigned, G...>) {
+        return (cc_term<Dict, E, G>(ms, cell) | ...);
+    }(std::make_integer_sequence<unsigned, 16>{});
+}
+
 // The reference's evaluator name: evaluator<word, dictionary, Expression>
 // (evaluators/bit_planes.hpp:48-49). Only 64-bit words are implemented on the GPU.
 template <typename cell_row_type, typename state_dictionary_type, typename Expression>
@@ -759,7 +877,13 @@ struct evaluator {
 
     template <class Win>
     CELLATO_HD static vint<planes> evaluate(const Win& w) {
-        return resize<planes>(as_vint(eval<state_dictionary_type, Expression>(w)));
+        if constexpr (planes == 1 && has_row_sums<Win>() && cc_rule<state_dictionary_type, Expression>::ok) {
+            vint<1> r;
+            r.p[0] = cc_eval<state_dictionary_type, Expression>(w);
+            return r;
+        } else {
+            return resize<planes>(as_vint(eval<state_dictionary_type, Expression>(w)));
+        }
     }
 };
 

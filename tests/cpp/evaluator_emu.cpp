@@ -12,6 +12,18 @@
 #include "layouts.hpp"
 #include "standard.hpp"
 
+// One-plane rules made of the cell and one Moore count (Game of Life, Maze) take the
+// evaluator's table path when the window has row sums (the row-sum cases below run it); the
+// other rules keep the node-by-node evaluation.
+namespace bp = cellato::evaluators::bit_planes;
+template <int id>
+constexpr bool table_path = cellato::automata::by_id<id>::config::state_dictionary::needed_bits == 1 &&
+                            bp::cc_rule<typename cellato::automata::by_id<id>::config::state_dictionary,
+                                        typename cellato::automata::by_id<id>::config::algorithm>::ok;
+static_assert(table_path<cellato::automata::GAME_OF_LIFE> && table_path<cellato::automata::MAZE>);
+static_assert(!table_path<cellato::automata::WIRE> && !table_path<cellato::automata::CRITTERS> &&
+              !table_path<cellato::automata::BRIAN> && !table_path<cellato::automata::CYCLIC>);
+
 using namespace cellato::evaluators::bit_planes;
 using u64 = std::uint64_t;
 
