@@ -499,7 +499,10 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
         rout[p] = plane_rsrc(a.out[p], plane_bytes);
     }
     const uint32_t col_off = static_cast<uint32_t>(col) * 8u;
-    const uint32_t halo_off = static_cast<uint32_t>(halo_col) * 8u;
+    // the end lanes' outside column; the other lanes read past the plane. The row offset is
+    // added with saturation and capped at OOB_OFFSET: no per-row select, which the compiler
+    // had turned into a branch around the load in every row.
+    const uint32_t halo_base = end_lane ? static_cast<uint32_t>(halo_col) * 8u : OOB_OFFSET;
 
     // generation-k rows, a ring of 3 slots per stage (linear rows carry their neighbours'
     // edge halves; tiled rows exchange whole words when used)
@@ -556,7 +559,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             hdst[p] = 0;
         }
         if constexpr (LOADS_HALO) {
-            const uint32_t hoff = end_lane ? roff + halo_off : OOB_OFFSET;
+            const uint32_t hoff = __builtin_elementwise_min(__builtin_elementwise_add_sat(roff, halo_base), OOB_OFFSET);
 #pragma unroll
             for (int p = 0; p < P; ++p) hdst[p] = buf_load(rin[p], hoff, 0u);
         }
@@ -643,7 +646,8 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))
             const int yo = s - T;
-            const uint32_t soff = yo >= r0 && yo < r1 && owner ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
+            const bool stored = (yo >= r0) & (yo < r1) & owner;   // no short circuit (see OOB_OFFSET)
+            const uint32_t soff = stored ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
 #pragma unroll
             for (int p = 0; p < P; ++p) buf_store(out.p[p], rout[p], soff, 0u);
         }
