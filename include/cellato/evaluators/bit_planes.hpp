@@ -440,6 +440,43 @@ struct ev<Dict, bit_or_<L, R>> {
     }
 };
 
+// (c ? a : b) | s per bit as one three-input op when a or b is a compile-time constant (a
+// rule's constant branch), instead of a select and an OR
+CELLATO_HD word select_or(word c, word a, word b, word s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_constant_p(a) && a == 0) return lop3<0xAE>(c, b, s);             // (~c & b) | s
+    if (__builtin_constant_p(a) && a == ~word(0)) return lop3<0xFE>(c, b, s);      // c | b | s
+    if (__builtin_constant_p(b) && b == 0) return lop3<0xEA>(c, a, s);             // (c & a) | s
+    if (__builtin_constant_p(b) && b == ~word(0)) return lop3<0xEF>(c, a, s);      // ~c | a | s
+#endif
+    return (a & c) | (b & ~c) | s;
+}
+
+// an OR whose left operand is an if-then-else (HPP: the vertical and horizontal results)
+template <class Dict, class C, class A, class B, class R>
+struct ev<Dict, bit_or_<if_then_else<C, A, B>, R>> {
+    template <class Win>
+    CELLATO_HD static auto at(const Win& w) {
+        const word c = eval<Dict, C>(w);
+        auto a = eval<Dict, A>(w);
+        auto b = eval<Dict, B>(w);
+        auto r = eval<Dict, R>(w);
+        if constexpr (std::is_same_v<decltype(a), word> && std::is_same_v<decltype(b), word> &&
+                      std::is_same_v<decltype(r), word>) {
+            return select_or(c, a, b, r);
+        } else {
+            const auto va = as_vint(a);
+            const auto vb = as_vint(b);
+            const auto vr = as_vint(r);
+            constexpr int N = cmax(cmax(decltype(va)::bits, decltype(vb)::bits), decltype(vr)::bits);
+            vint<N> out;
+#pragma unroll
+            for (int i = 0; i < N; ++i) out.p[i] = select_or(c, plane(va, i), plane(vb, i), plane(vr, i));
+            return out;
+        }
+    }
+};
+
 template <class Dict, class L, class R>
 struct ev<Dict, plus<L, R>> {
     template <class Win>
