@@ -572,6 +572,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
 #pragma unroll
     for (int u = 0; u < D; ++u) load_row(pf[u], hpf[u]);
     int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
+    int kmod[T];
+#pragma unroll
+    for (int k = 0; k < T; ++k) kmod[k] = k % a.h;
 
     // One block of U steps. FILL: the block may fall in the pipeline fill (steps below 2T),
     // where stages that no stored row needs are skipped.
@@ -617,8 +620,11 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 if (FILL && ib + u < 2 * k + 2) continue;
                 const int t = TPAR < 0 ? a.t0 + k : (TPAR + k) & 1;
                 if constexpr (LAYOUT == LINEAR) {
-                    int y = ycur - k;          // only rules with a Margolus block read it
-                    while (y < 0) y += a.h;
+                    // the row of stage k (only rules with a Margolus block read it): one
+                    // conditional add, k mod h being a loop-invariant scalar (a loop here made
+                    // every stage branch)
+                    int y = ycur - kmod[k];
+                    y += y < 0 ? a.h : 0;
                     const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
                     out = Eval::evaluate(wnd);
                 } else {
