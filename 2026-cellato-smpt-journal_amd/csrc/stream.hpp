@@ -338,9 +338,11 @@ __device__ __forceinline__ int wrap(int v, int m) {
     return r < 0 ? r + m : r;
 }
 
-// Plane access through buffer resources: the row offset is a scalar (SGPR) operand and the
-// column offset a per-lane constant, so a load or store costs no address arithmetic; an
-// offset past the plane reads 0 instead of faulting. Planes are < 4 GiB (checked on the host).
+// Plane access through buffer resources with 32-bit offsets: a row load of a whole-segment
+// wave takes the row offset as a scalar (SGPR) operand and the column offset as a per-lane
+// constant (no address arithmetic); lane groups on different rows, stores and the outside
+// columns pass one per-lane offset (one add). An offset past the plane reads 0 instead of
+// faulting. Planes are < 4 GiB (checked on the host).
 constexpr int BUFFER_DWORD3 = 0x00020000;   // raw, untyped 32-bit-element access on gfx9
 // An offset past every plane (planes hold at most 2^32 - 8 bytes): a load there reads 0 and a
 // store there is dropped. Lanes that have nothing to load or store issue the instruction with
