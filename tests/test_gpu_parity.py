@@ -69,3 +69,36 @@ def test_published_checksum_16384(rule):
     for layout, kind, T in [(0, "simple", 1), (0, "temporal", 8), (1, "temporal", 4)]:
         out = run_gpu(rule, init, layout, kind, T, 128)
         assert O.checksum(out) == g["checksum"], (g["source"], layout, kind, T)
+
+
+@pytest.mark.parametrize("rule", ["game-of-life", "wire"])
+def test_large_grid_paths_agree(rule):
+    """Past the Infinity Cache and past 2^28-byte plane offsets (32768^2: 128 MiB per plane):
+    the simple traverser (64-column segments, loaded outside columns) and the temporal one at
+    depths 3 and 8 (62-column segments with halo lanes, tail lane groups) give the same grid
+    after 16 generations from the same device-made state; depth 1 and the simple traverser
+    are pinned to the oracle above, so the three agreeing pins the large-grid runs too."""
+    import torch
+    import cellato_amd as C
+    from cellato_amd import _lib
+    from cellato_amd.planes import DeviceGrid
+    r = C.rule(rule)
+    n, steps = 32768, 16
+    stream = torch.cuda.current_stream().cuda_stream
+    results = []
+    for kind, T in (("simple", 1), ("temporal", 3), ("temporal", 8)):
+        dev = DeviceGrid(r, n, n, 0)
+        dev.fill_random(5)
+        if kind == "simple":
+            in_b = _lib.simple(r.id, 0, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), n, n, 0, steps, 0, stream)
+        else:
+            in_b = _lib.temporal(r.id, 0, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), n, n, 0, steps, T, 0, stream)
+        if in_b:
+            dev.cur = 1 - dev.cur
+        torch.cuda.synchronize()
+        results.append(dev.buf[dev.cur].clone())
+        del dev
+        torch.cuda.empty_cache()
+    assert bool(results[0].any()), f"{rule}: empty grid after {steps} generations"
+    assert torch.equal(results[0], results[1]), f"{rule}: simple != temporal T=3 at {n}^2"
+    assert torch.equal(results[0], results[2]), f"{rule}: simple != temporal T=8 at {n}^2"
