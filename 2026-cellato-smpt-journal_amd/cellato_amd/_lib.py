@@ -32,6 +32,15 @@ def plan(rule_id, layout, width, height, temporal_steps=0, chunk_rows=0):
     return c.value, t.value, b.value
 
 
+def kernel_info(rule_id, layout, width, temporal_steps=0, batch=False):
+    """The kernel a pass would launch (needs a device): (depth it runs at, scratch bytes per
+    lane, registers per lane). temporal_steps 0 = the simple traverser."""
+    t, p, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib().cellato_hip_kernel_info(rule_id, layout, width, temporal_steps, int(batch), ctypes.byref(t),
+                                        ctypes.byref(p), ctypes.byref(r)))
+    return t.value, p.value, r.value
+
+
 class CellatoError(RuntimeError):
     """A failed C-ABI call: `code` is the negative status, the message comes from the library."""
 
@@ -81,6 +90,7 @@ def lib():
                                           ctypes.POINTER(ctypes.c_longlong)]
         L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
         L.cellato_hip_plan.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.cellato_hip_kernel_info.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)] * 3
         L.cellato_batch_plan.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]

@@ -105,9 +105,11 @@ int plan_schedule(schedule& S, const cellato_batch_grid* grids, int n, int max_w
         const int P = kernels_of_rule(g.rule).planes;
         for (int p = 0; p < P; ++p)
             if (!g.a[p] || !g.b[p]) return bfail(CELLATO_E_ARG, "null plane pointer");
+        // a depth whose batch kernel would need scratch memory is stepped down
+        const int Tg = usable_depth(g.rule, g.layout, g.width / 64, g.temporal_steps, true);
         int done = 0, r = 0;
         while (done < g.steps) {
-            const int T = std::min(g.temporal_steps, g.steps - done);
+            const int T = std::min(Tg, g.steps - done);
             chain[gi].push_back({gi, T, done, r});
             done += T;
             ++r;

@@ -65,12 +65,14 @@ int launch(const pass_plan& pl, hipStream_t stream) {
 int run_passes(int rule, int layout, uint64_t* const* a, uint64_t* const* b, int width, int height, int t0,
                int steps, int T, bool temporal, int chunk, hipStream_t stream, int* result_in_b) {
     const int P = kernels_of_rule(rule).planes;
+    // a depth whose kernel would need scratch memory runs as passes of the deepest one that does not
+    if (temporal) T = usable_depth(rule, layout, words_per_row(layout, width), T);
     uint64_t* const* cur = a;
     uint64_t* const* nxt = b;
     int flips = 0;
     int done = 0;
     while (done < steps) {
-        const int g = steps - done < T ? steps - done : T;
+        const int g = temporal ? usable_depth(rule, layout, words_per_row(layout, width), steps - done < T ? steps - done : T) : 1;
         pass_plan pl;
         const int rc = plan_pass(rule, layout, width, height, temporal ? g : 1, temporal, chunk, pl);
         if (rc != CELLATO_OK) return fail(rc, plan_error());
@@ -161,11 +163,39 @@ int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_s
     if (temporal_steps < 0 || temporal_steps > MAX_T) return fail(CELLATO_E_TEMPORAL, "temporal_steps out of range");
     pass_plan pl;
     const bool temporal = temporal_steps > 0;
-    const int rc = plan_pass(rule, layout, width, height, temporal ? temporal_steps : 1, temporal, chunk_rows, pl);
+    const int T = temporal ? usable_depth(rule, layout, words_per_row(layout, width), temporal_steps) : 1;
+    const int rc = plan_pass(rule, layout, width, height, T, temporal, chunk_rows, pl);
     if (rc != CELLATO_OK) return fail(rc, plan_error());
     if (chunk) *chunk = pl.args.chunk;
     if (tasks) *tasks = pl.args.ntasks;
     if (blocks) *blocks = pl.blocks;
+    return CELLATO_OK;
+}
+
+int cellato_hip_kernel_info(int rule, int layout, int width, int temporal_steps, int batch, int* effective_steps,
+                            int* private_segment_bytes, int* registers) {
+    if (rule < 0 || rule >= RULES) return fail(CELLATO_E_RULE, "unknown rule id " + std::to_string(rule));
+    if (layout != CELLATO_LAYOUT_LINEAR && layout != CELLATO_LAYOUT_TILED) return fail(CELLATO_E_LAYOUT, "unknown layout");
+    if (batch && layout != CELLATO_LAYOUT_LINEAR) return fail(CELLATO_E_LAYOUT, "batches use the linear layout");
+    if (width <= 0 || (layout == CELLATO_LAYOUT_LINEAR && width % 64) || (layout == CELLATO_LAYOUT_TILED && width % 8))
+        return fail(CELLATO_E_GEOMETRY, "grid width not representable in the layout");
+    if (temporal_steps < 0 || temporal_steps > MAX_T || (batch && temporal_steps < 1))
+        return fail(CELLATO_E_TEMPORAL, "temporal_steps out of range");
+    const rule_kernels& k = kernels_of_rule(rule);
+    const int wx = words_per_row(layout, width);
+    const bool temporal = temporal_steps > 0;
+    const int T = temporal ? usable_depth(rule, layout, wx, temporal_steps, batch != 0) : 1;
+    const void* fn = batch ? k.batch[T] : pass_kernel(k, layout, wx, T, temporal);
+    if (!fn) return fail(CELLATO_E_TEMPORAL, "no kernel instantiated for this configuration");
+    hipFuncAttributes attr{};
+    const hipError_t e = hipFuncGetAttributes(&attr, fn);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(CELLATO_E_HIP, std::string("hipFuncGetAttributes: ") + hipGetErrorString(e));
+    }
+    if (effective_steps) *effective_steps = temporal ? T : 0;
+    if (private_segment_bytes) *private_segment_bytes = static_cast<int>(attr.localSizeBytes);
+    if (registers) *registers = attr.numRegs;
     return CELLATO_OK;
 }
 

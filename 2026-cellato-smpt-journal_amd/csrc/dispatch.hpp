@@ -38,7 +38,9 @@ struct rule_kernels {
     const void* batch[MAX_T + 1];                             // linear layout, [T], T >= 1
 };
 
-template <class Eval, int L>
+// MaxT: the deepest temporal kernel instantiated for this layout; deeper entries stay null and
+// the planner steps a pass down to the deepest depth that exists and uses no scratch memory.
+template <class Eval, int L, int MaxT = MAX_T>
 constexpr void fill_layout(rule_kernels& k) {
     k.simple[L][SIMPLE_SEGMENT] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_SEGMENT, HALO_LOAD>);
     k.simple[L][SIMPLE_RING64] = reinterpret_cast<const void*>(&stream_kernel<Eval, L, 1, XCH_RING64, HALO_NONE>);
@@ -46,6 +48,8 @@ constexpr void fill_layout(rule_kernels& k) {
     k.temporal[L][TEMPORAL_SEGMENT][0] = nullptr;
     k.temporal[L][TEMPORAL_RING64][0] = nullptr;
     k.temporal[L][TEMPORAL_GROUPS][0] = nullptr;
+    for (int v = 0; v < TEMPORAL_VARIANTS; ++v)
+        for (int t = MaxT + 1; t <= MAX_T; ++t) k.temporal[L][v][t] = nullptr;
     [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
         ((k.temporal[L][TEMPORAL_SEGMENT][Ts + 1] =
               reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_SEGMENT, HALO_LANES>)), ...);
@@ -53,10 +57,13 @@ constexpr void fill_layout(rule_kernels& k) {
               reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_RING64, HALO_NONE>)), ...);
         ((k.temporal[L][TEMPORAL_GROUPS][Ts + 1] =
               reinterpret_cast<const void*>(&stream_kernel<Eval, L, Ts + 1, XCH_PERMUTE, HALO_NONE>)), ...);
-    }(std::make_integer_sequence<int, MAX_T>{});
+    }(std::make_integer_sequence<int, MaxT>{});
 }
 
-template <class Config>
+// MaxLinearT: the deepest depth whose linear-layout kernels fit the register file without
+// scratch memory (the linear ring slots also hold the neighbours' edge halves, so a rule with
+// many planes runs out of VGPRs at a lower depth than in the tiled layout).
+template <class Config, int MaxLinearT = MAX_T>
 rule_kernels make_rule_kernels() {
     using Eval = cellato::evaluators::bit_planes::evaluator<word, typename Config::state_dictionary, typename Config::algorithm>;
     rule_kernels k{};
@@ -66,12 +73,12 @@ rule_kernels make_rule_kernels() {
     k.traits = cellato::evaluators::bit_planes::rule_traits<typename Config::algorithm>::value;
     for (int i = 0; i < k.nstates && i < 32; ++i)
         k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
-    fill_layout<Eval, LINEAR>(k);
+    fill_layout<Eval, LINEAR, MaxLinearT>(k);
     fill_layout<Eval, TILED>(k);
-    k.batch[0] = nullptr;
+    for (int t = 0; t <= MAX_T; ++t) k.batch[t] = nullptr;
     [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
         ((k.batch[Ts + 1] = reinterpret_cast<const void*>(&batch_kernel<Eval, LINEAR, Ts + 1>)), ...);
-    }(std::make_integer_sequence<int, MAX_T>{});
+    }(std::make_integer_sequence<int, MaxLinearT>{});
     return k;
 }
 

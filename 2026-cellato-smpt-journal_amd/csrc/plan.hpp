@@ -64,6 +64,44 @@ inline int blocks_per_cu(const void* fn) {
     return n;
 }
 
+// Bytes of scratch (private segment) a kernel needs per lane: nonzero means its registers
+// spilled to memory, which multiplies its HBM traffic. Cached per kernel; 0 when it cannot
+// be asked (no device: the plans made on a host without a GPU).
+inline int private_bytes(const void* fn) {
+    static std::mutex mu;
+    static std::map<const void*, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(fn);
+    if (it != cache.end()) return it->second;
+    hipFuncAttributes attr{};
+    int n = 0;
+    if (hipFuncGetAttributes(&attr, fn) == hipSuccess) n = static_cast<int>(attr.localSizeBytes);
+    else (void)hipGetLastError();
+    cache[fn] = n;
+    return n;
+}
+
+// words per row of a grid `width` cells wide (linear: 64 cells per word; tiled: 8)
+inline int words_per_row(int layout, int width) { return layout == CELLATO_LAYOUT_LINEAR ? width / 64 : width / 8; }
+
+// Kernel of one pass: the stream kernel of the lane mapping that a row of wx words takes.
+inline const void* pass_kernel(const rule_kernels& k, int layout, int wx, int T, bool temporal) {
+    if (!temporal) return k.simple[layout][wx < 64 ? SIMPLE_GROUPS : wx == 64 ? SIMPLE_RING64 : SIMPLE_SEGMENT];
+    return k.temporal[layout][wx < 64 ? TEMPORAL_GROUPS : wx == 64 ? TEMPORAL_RING64 : TEMPORAL_SEGMENT][T];
+}
+
+// The depth a pass of up to T generations runs at: the deepest T' <= T whose kernel exists
+// and needs no scratch memory (kernels that would spill are not instantiated, and any that
+// spill on this device are stepped over). `batch`: the work-list kernel of batches instead.
+inline int usable_depth(int rule, int layout, int wx, int T, bool batch = false) {
+    const rule_kernels& k = kernels_of_rule(rule);
+    for (int t = T; t > 1; --t) {
+        const void* fn = batch ? k.batch[t] : pass_kernel(k, layout, wx, t, true);
+        if (fn && private_bytes(fn) == 0) return t;
+    }
+    return 1;
+}
+
 // Rows per task when the caller passes 0. Every wave runs one task, and a CU takes its
 // blocks (4 waves, one per SIMD) at once, so a launch lasts about m x (rows a task walks),
 // m = blocks on the busiest CU; a task walks chunk + 2T rows rounded up to whole unrolled
@@ -163,14 +201,19 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         return a.nseg * nch + (a.tail_n > 0 ? (h + static_cast<long long>(a.tail_groups) * c - 1) /
                                                    (static_cast<long long>(a.tail_groups) * c) : 0);
     };
-    a.chunk = chunk > 0 ? chunk : auto_chunk(tasks_of, T, rows_per_group, blocks_per_cu(pl.fn));
-    a.nchunk = (h + a.groups * a.chunk - 1) / (a.groups * a.chunk);
-    const long long main_tasks = static_cast<long long>(a.nseg) * a.nchunk;
-    const long long tasks = main_tasks + (a.tail_n > 0 ? (h + a.tail_groups * a.chunk - 1) / (a.tail_groups * a.chunk) : 0);
+    // a caller's chunk is capped at the rows a lane group covers (a longer one would only make
+    // every wave walk the wrapped rows again)
+    a.chunk = chunk > 0 ? (chunk < rows_per_group ? chunk : rows_per_group)
+                        : auto_chunk(tasks_of, T, rows_per_group, blocks_per_cu(pl.fn));
+    const long long nchunk = (h + static_cast<long long>(a.groups) * a.chunk - 1) / (static_cast<long long>(a.groups) * a.chunk);
+    const long long main_tasks = static_cast<long long>(a.nseg) * nchunk;
+    const long long tasks = main_tasks + (a.tail_n > 0 ? (h + static_cast<long long>(a.tail_groups) * a.chunk - 1) /
+                                                             (static_cast<long long>(a.tail_groups) * a.chunk) : 0);
     if (tasks > (1LL << 30)) {
         plan_error() = "grid too large for one launch";
         return CELLATO_E_GEOMETRY;
     }
+    a.nchunk = static_cast<int>(nchunk);
     a.ntasks = static_cast<int>(tasks);
     a.main_tasks = static_cast<int>(main_tasks);
     pl.blocks = (a.ntasks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;

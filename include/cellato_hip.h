@@ -63,7 +63,9 @@ int cellato_max_temporal_steps(void);
  * simple:   one generation per kernel launch.
  * temporal: temporal_steps generations per kernel launch (register-resident pipeline);
  *           steps need not be a multiple of temporal_steps (the remainder runs as a
- *           shorter last launch).
+ *           shorter last launch). A depth whose kernel would need scratch memory (the
+ *           cyclic automaton's linear layout above 4) runs as launches of the deepest depth
+ *           that does not; cellato_hip_kernel_info reports the depth used.
  * Geometry: linear needs width % 64 == 0; tiled needs width % 8 == 0 and height % 8 == 0. */
 int cellato_hip_simple(int rule, int layout, uint64_t* const* buf_a, uint64_t* const* buf_b,
                        int width, int height, int t0, int steps, int chunk_rows,
@@ -79,6 +81,15 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
  * reference (whose launch shape is fixed by its tile-size parameters). */
 int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_steps, int chunk_rows, int* chunk,
                      int* tasks, int* blocks);
+
+/* The kernel a pass would launch, described without launching it (needs a device): the
+ * depth it runs at (*effective_steps; temporal_steps 0 = the simple traverser, reported as 0),
+ * the bytes of scratch memory per lane it needs (*private_segment_bytes) and its registers per
+ * lane. A temporal depth whose kernel would need scratch memory (registers spilled to memory)
+ * runs as passes of the deepest depth whose kernel does not; `batch` != 0 asks for the batch
+ * work-list kernel of that depth (linear layout). No counterpart in the reference. */
+int cellato_hip_kernel_info(int rule, int layout, int width, int temporal_steps, int batch, int* effective_steps,
+                            int* private_segment_bytes, int* registers);
 
 /* ---- batches ------------------------------------------------------------------------------
  * Many independent grids (any of the rules, any size, own step count, start step and depth)
