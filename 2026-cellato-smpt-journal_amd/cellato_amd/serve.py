@@ -108,16 +108,62 @@ def admit(waiting, n_live, used, budget, max_live):
     return admitted, refused, used
 
 
+class GpuEngine:
+    """The device side of the serving loop: ticks go to `cellato_hip_multi` on the current
+    stream, completion times come from HIP events against an event recorded at start."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.stream = torch.cuda.current_stream()
+        self.start_ev = None
+
+    def start(self):
+        self.torch.cuda.synchronize()
+        self.start_ev = self.torch.cuda.Event(enable_timing=True)
+        self.start_ev.record(self.stream)
+
+    def tick(self, arr, n):
+        """Advance the n grids of `arr` (cellato_batch_grid array); returns result_in_b per grid."""
+        in_b = (ctypes.c_int * n)()
+        _lib.check(_lib.lib().cellato_hip_multi(arr, n, 0, ctypes.c_void_p(self.stream.cuda_stream), in_b))
+        return list(in_b)
+
+    def record(self):
+        ev = self.torch.cuda.Event(enable_timing=True)
+        ev.record(self.stream)
+        return ev
+
+    def done(self, ev):
+        return ev.query()
+
+    def wait(self, ev):
+        ev.synchronize()
+
+    def seconds(self, ev):
+        """GPU time of a recorded event after start(), in seconds."""
+        return self.start_ev.elapsed_time(ev) / 1e3
+
+
 class Server:
+    """Continuous batching of simulation requests under an HBM budget.
+
+    A job's two buffers count against the budget from its admission until the tick that
+    finishes it has retired on the GPU (its grid is referenced by the launches in flight until
+    then); only then are its bytes given to new requests. `allocate(request)` makes a job's
+    grid and `release(grid)` drops it (a test may pass fakes for both, together with a fake
+    `engine`, to follow the bytes held)."""
+
     def __init__(self, hbm_budget=200e9, max_live=512, tuner=None, default_T=4, keep_results=False,
-                 allocate=None, tick_passes=4):
+                 allocate=None, tick_passes=4, release=None, engine=None):
         self.budget, self.max_live = hbm_budget, max_live
         self.tick_passes = tick_passes
         self.tuner, self.default_T = tuner, default_T
         self.keep_results = keep_results
         self.results = {}       # request id -> final DeviceGrid (keep_results)
-        # allocate(request) -> DeviceGrid; a test may pass a fake to exercise admission
         self.allocate = allocate or (lambda r: DeviceGrid(get_rule(r.rule), r.width, r.height, 0))
+        self.release = release or (lambda grid: None)
+        self.engine = engine
 
     def _config(self, r):
         if self.tuner is not None:
@@ -125,17 +171,14 @@ class Server:
         return self.default_T, 0
 
     def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
-        import torch
+        eng = self.engine or GpuEngine()
         rep = ServeReport()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         live = []
-        used = 0
-        stream = torch.cuda.current_stream()
-        start_ev = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
+        used = 0                # bytes of every admitted job whose last tick has not retired
+        eng.start()
         t0 = clock()
-        start_ev.record(stream)
         inflight = []           # (event, [live jobs finishing with this tick])
         while pending or waiting or live or inflight:
             now = clock() - t0
@@ -163,8 +206,7 @@ class Server:
                         g.a[p] = ptr
                     for p, ptr in enumerate(j.grid.ptrs(1 - j.grid.cur)):
                         g.b[p] = ptr
-                in_b = (ctypes.c_int * len(live))()
-                _lib.check(_lib.lib().cellato_hip_multi(arr, len(live), 0, ctypes.c_void_p(stream.cuda_stream), in_b))
+                in_b = eng.tick(arr, len(live))
                 rep.ticks += 1
                 finished, still = [], []
                 for i, j in enumerate(live):
@@ -172,23 +214,24 @@ class Server:
                         j.grid.cur = 1 - j.grid.cur
                     j.done += arr[i].steps
                     (finished if j.done >= j.req.steps else still).append(j)
-                ev = torch.cuda.Event(enable_timing=True)
-                ev.record(stream)
-                inflight.append((ev, finished))
+                inflight.append((eng.record(), finished))
                 live = still
-                for j in finished:
-                    used -= j.req.bytes
-            # retire ticks whose GPU work is done (keep at most two in flight)
-            while inflight and (len(inflight) > 2 or inflight[0][0].query() or not live):
+            # retire ticks whose GPU work is done (keep at most two in flight); a finished job's
+            # bytes return to the budget here, once the launches that use its grid are over
+            while inflight and (len(inflight) > 2 or eng.done(inflight[0][0]) or not live):
                 ev, fin = inflight.pop(0)
-                ev.synchronize()
-                t_done = start_ev.elapsed_time(ev) / 1e3
+                eng.wait(ev)
+                t_done = eng.seconds(ev)
                 for j in fin:
                     rep.completions[j.req.id] = t_done
                     rep.latencies[j.req.id] = t_done - j.req.arrival
                     rep.cell_updates += j.req.cell_updates
+                    used -= j.req.bytes
                     if self.keep_results:
                         self.results[j.req.id] = j.grid
+                    else:
+                        self.release(j.grid)
+                    j.grid = None
             if not live and not waiting and pending and not inflight:
                 sleep(max(0.0, min(0.01, pending[0].arrival - (clock() - t0))))
         rep.makespan = max(rep.completions.values()) - min(r.arrival for r in requests) if rep.completions else 0.0

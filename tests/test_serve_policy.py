@@ -31,3 +31,71 @@ def test_workload_shape():
     arr = [r.arrival for r in reqs]
     assert arr == sorted(arr) and abs(len(reqs) / arr[-1] - 4.0) < 1.0
     assert max(r.bytes for r in reqs) <= 2 * 5 * 8192 * 8192 // 8
+
+
+class FakeEngine:
+    """Ticks that stay in flight for `lag` polls of done() after being recorded, as GPU work
+    does while the host runs ahead; times are tick counts."""
+
+    def __init__(self, lag=3):
+        self.lag, self.ticks, self.start_called = lag, 0, False
+
+    def start(self):
+        self.start_called = True
+
+    def tick(self, arr, n):
+        self.ticks += 1
+        return [arr[i].steps % 2 for i in range(n)]   # one pass per generation: odd counts flip
+
+    def record(self):
+        return {"tick": self.ticks, "polls": 0, "done": False}
+
+    def done(self, ev):
+        ev["polls"] += 1
+        ev["done"] = ev["done"] or ev["polls"] > self.lag
+        return ev["done"]
+
+    def wait(self, ev):
+        ev["done"] = True
+
+    def seconds(self, ev):
+        return float(ev["tick"])
+
+
+class FakeGrid:
+    def __init__(self, r):
+        from cellato_amd.rules import rule as get_rule
+        self.rule, self.cur, self.bytes = get_rule(r.rule), 0, r.bytes
+
+    def fill_random(self, seed):
+        pass
+
+    def ptrs(self, which):
+        return [0x1000 * (which + 1)] * self.rule.planes
+
+
+def test_budget_holds_until_the_finishing_tick_retires():
+    """The bytes of grids actually alive (allocated and not yet released) never exceed the
+    budget, although ticks stay in flight after their jobs finish on the host."""
+    from cellato_amd.serve import Server
+    reqs = [Request(i, "game-of-life" if i % 2 else "wire", 64, 64, 4 + 4 * (i % 3), 0.0, i, 100)
+            for i in range(40)]
+    alive, peak = [0], [0]
+
+    def allocate(r):
+        alive[0] += r.bytes
+        peak[0] = max(peak[0], alive[0])
+        assert alive[0] <= 350, alive[0]
+        return FakeGrid(r)
+
+    def release(g):
+        alive[0] -= g.bytes
+
+    eng = FakeEngine(lag=3)
+    srv = Server(hbm_budget=350, max_live=64, default_T=4, tick_passes=1, allocate=allocate, release=release,
+                 engine=eng)
+    rep = srv.run(reqs, clock=lambda: 1.0, sleep=lambda s: None)
+    assert eng.start_called and len(rep.completions) == 40
+    assert alive[0] == 0 and peak[0] == 300 and rep.peak_bytes <= 350
+    # completion order follows the ticks: a job finishes in the tick of its last pass
+    assert all(rep.completions[r.id] >= r.steps // 4 for r in reqs)
