@@ -905,6 +905,9 @@ struct evaluator {
     using dictionary = state_dictionary_type;
     using expression = Expression;
     static constexpr int planes = state_dictionary_type::needed_bits;
+    // grid layout the evaluator reads (0 linear, 1 tiled 8x8): the HIP traversers check that
+    // it matches the grid's
+    static constexpr int layout = 0;
     // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of
     using row_sum_targets = typename row_targets<state_dictionary_type, Expression>::type;
     // the rule reads the parity of the time step (alternating algorithms, Margolus blocks):

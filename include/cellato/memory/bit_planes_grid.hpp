@@ -1,10 +1,11 @@
 // Bit-plane grids for the HIP traversers: the reference's bit_planes::grid and
 // tiled_bit_planes::grid (include/memory/bit_planes_grid.hpp:30-271,
-// tiled_bit_planes_grid.hpp:30-289) with the same members (needed_bits, x/y_word_tile_size,
-// store_type, data(), x_size_physical(), y_size_physical(), to_standard(), get_cell(),
-// free_cuda_memory() under the name free_device_memory()). Differences: the device copy
-// lives in one hipMalloc'd block (planes back to back) and to_hip()/to_cpu() replace
-// to_cuda()/to_cpu(). Host code only; needs the HIP runtime library.
+// tiled_bit_planes_grid.hpp:30-289) with the same members (needed_bits, word_store_bits,
+// x/y_word_tile_size, store_type, data(), x_size_physical(), y_size_physical(), to_standard(),
+// get_cell(), to_cuda(), to_cpu(), free_cuda_memory(), and the implicit conversion from a
+// standard grid that the reference's experiment manager relies on). The device copy lives in
+// one hipMalloc'd block (planes back to back); to_hip() / free_device_memory() are the same
+// calls under this library's names. Host code only; needs the HIP runtime library.
 #ifndef CELLATO_MEMORY_BIT_PLANES_GRID_HPP
 #define CELLATO_MEMORY_BIT_PLANES_GRID_HPP
 
@@ -36,6 +37,7 @@ class plane_grid {
 public:
     static_assert(sizeof(word_t) == 8, "64-bit words");
     static constexpr int needed_bits = dict_t::needed_bits;
+    static constexpr int word_store_bits = sizeof(word_t) * 8;
     static constexpr int x_word_tile_size = Tiled ? 8 : 64;
     static constexpr int y_word_tile_size = Tiled ? 8 : 1;
     static constexpr int layout = Tiled ? 1 : 0;
@@ -46,8 +48,8 @@ public:
 
     plane_grid() = default;
 
-    // from a standard grid of state values (bit_planes_grid.hpp:80-84)
-    explicit plane_grid(const standard::grid<original_state_t>& s) requires (dev == device::CPU)
+    // from a standard grid of state values (bit_planes_grid.hpp:80-84); implicit, as there
+    plane_grid(const standard::grid<original_state_t>& s) requires (dev == device::CPU)
         : plane_grid(s.y_size_physical(), s.x_size_physical(), s.data()) {}
 
     plane_grid(std::size_t y_size, std::size_t x_size, const original_state_t* cells) requires (dev == device::CPU) {
@@ -113,12 +115,17 @@ public:
         return g;
     }
 
+    // the reference's name for the device copy (bit_planes_grid.hpp:132)
+    plane_grid<word_t, dict_t, Tiled, device::HIP> to_cuda() const requires (dev == device::CPU) { return to_hip(); }
+
     void free_device_memory() {
         if constexpr (dev == device::HIP) {
             if (_dev) hip_check(hipFree(_dev), "hipFree");
             _dev = nullptr;
         }
     }
+    // the reference's name (bit_planes_grid.hpp:86-95)
+    void free_cuda_memory() { free_device_memory(); }
 
     std::string get_checksum() const requires (dev == device::CPU) { return to_standard().get_checksum(); }
 

@@ -1,8 +1,14 @@
 // The HIP traversers as C++ templates with the reference's interface
 // (include/traversers/cuda/simple.hpp:25-106, temporal.hpp:24-150):
 //
-//   traverser<evaluator, grid> t;  t.init(grid, params);  t.run(steps[, callback]);
-//   grid result = t.fetch_result();
+//   simple::traverser<evaluator, grid> t;                          (simple.hpp:25-28)
+//   temporal::traverser<evaluator, grid, average_halo_radius> t;   (temporal.hpp:24-27)
+//   t.init(grid, params);  t.run(steps[, callback]);  grid result = t.fetch_result();
+//
+// so a test suite of the reference (include/experiments/test_suites.hpp:116-225) binds them
+// with its namespace alias changed from cellato::traversers::cuda to cellato::traversers::hip.
+// The evaluator is bit_planes::evaluator for the linear grid, tiled_bit_planes::evaluator for
+// the tiled one (checked at compile time: a mismatch would compute a different automaton).
 //
 // Each call goes through the C ABI of include/cellato_hip.h (libcellato_hip.so), which holds
 // the gfx950 kernels of the automata registered in include/cellato/automata/all.hpp; the
@@ -18,6 +24,8 @@
 
 #include "../../../cellato_hip.h"
 #include "../../automata/all.hpp"
+#include "../../evaluators/bit_planes.hpp"
+#include "../../evaluators/tiled_bit_planes.hpp"
 #include "../../experiments/run_params.hpp"
 #include "../../memory/bit_planes_grid.hpp"
 
@@ -38,14 +46,22 @@ inline void check(int rc) {
     if (rc < 0) throw std::runtime_error(cellato_hip_last_error());
 }
 
-template <typename evaluator_type, typename grid_type, bool Temporal>
+// average_halo_radius: cells of halo one generation needs (the reference sizes its temporal
+// tiles with it, temporal.hpp:47); the kernels here carry a one-cell halo per generation,
+// which covers every rule whose neighbour reads stay within distance 1.
+template <typename evaluator_type, typename grid_type, bool Temporal, double average_halo_radius = 1.0>
 class traverser_base {
     using evaluator_t = evaluator_type;
     using grid_t = grid_type;
     static constexpr int P = grid_t::needed_bits;
     static constexpr int rule = rule_id_of<typename evaluator_t::expression>();
     static_assert(rule >= 0, "this rule is not compiled into libcellato_hip.so (include/cellato/automata/all.hpp)");
-    using device_grid_t = std::invoke_result_t<decltype(&grid_t::to_hip), grid_t>;
+    static_assert(evaluator_t::layout == grid_t::layout,
+                  "evaluator and grid layouts differ: bit_planes::evaluator goes with bit_planes::grid, "
+                  "tiled_bit_planes::evaluator with tiled_bit_planes::grid");
+    static_assert(average_halo_radius > 0.0 && average_halo_radius <= 1.0,
+                  "the HIP kernels carry a halo of one cell per generation");
+    using device_grid_t = std::invoke_result_t<decltype(&grid_t::to_cuda), grid_t>;
 
 public:
     static constexpr bool is_CUDA = true;   // the reference's name for "runs on the GPU"
@@ -59,8 +75,8 @@ public:
                 throw std::runtime_error("Invalid temporal steps for the temporal traverser.");
         }
         _params = params;
-        _cur = grid.to_hip();
-        _other = grid.to_hip();
+        _cur = grid.to_cuda();
+        _other = grid.to_cuda();
         _t = 0;
         _ready = true;
     }
@@ -84,8 +100,8 @@ public:
     grid_t fetch_result() {
         cellato::memory::grids::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         grid_t r = _cur.to_cpu();
-        _cur.free_device_memory();
-        _other.free_device_memory();
+        _cur.free_cuda_memory();
+        _other.free_cuda_memory();
         _ready = false;
         return r;
     }
@@ -117,8 +133,8 @@ template <typename evaluator_type, typename grid_type>
 using traverser = traverser_base<evaluator_type, grid_type, false>;
 }
 namespace temporal {
-template <typename evaluator_type, typename grid_type>
-using traverser = traverser_base<evaluator_type, grid_type, true>;
+template <typename evaluator_type, typename grid_type, double average_halo_radius = 1.0>
+using traverser = traverser_base<evaluator_type, grid_type, true, average_halo_radius>;
 }
 
 } // namespace cellato::traversers::hip
