@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -330,12 +331,84 @@ int run_multi(const cellato_batch_grid* grids, int n, int max_waves, hipStream_t
     return CELLATO_OK;
 }
 
+// ---- cellato_hip_multi_static: the same schedule with its tables in static device memory ----
+// The pass table and task lists of one call live in `scratch` (file scope, below), a device
+// array that each GPU holds for the life of the process (one copy per device). The host's name
+// `scratch` is not a device address: the address of the current device's copy comes from
+// hipGetSymbolAddress, taken once per device on the first call there, and only that address is
+// passed to copies and kernels. Every call on a device shares its copy: calls that overlap on
+// one GPU (other streams, other host threads) race.
+constexpr size_t STATIC_SCRATCH_BYTES = CELLATO_MULTI_STATIC_SCRATCH_BYTES;
+static_assert(sizeof(pass_args) == 144 && sizeof(batch_task) == 8, "table sizes stated in cellato_hip.h");
+
+} // namespace
+} // namespace cellato::hip
+
+static __device__ __align__(256) unsigned char scratch[cellato::hip::STATIC_SCRATCH_BYTES];
+
+namespace cellato::hip {
+namespace {
+
+int static_scratch(void** out) {
+    static std::mutex mu;
+    static std::vector<void*> by_device;
+    int dev = 0;
+    BCHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (static_cast<int>(by_device.size()) <= dev) by_device.resize(dev + 1, nullptr);
+    if (!by_device[dev]) {
+        void* p = nullptr;
+        BCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        by_device[dev] = p;
+    }
+    *out = by_device[dev];
+    return CELLATO_OK;
+}
+
+int run_multi_static(const cellato_batch_grid* grids, int n, int max_waves, hipStream_t stream, int* result_in_b) {
+    schedule S;
+    const int prc = plan_schedule(S, grids, n, max_waves);
+    if (prc != CELLATO_OK) return prc;
+    const size_t pbytes = ((S.passes.size() * sizeof(pass_args) + 255) / 256) * 256;
+    const size_t bytes = pbytes + S.tasks.size() * sizeof(batch_task);
+    if (bytes > STATIC_SCRATCH_BYTES)
+        return bfail(CELLATO_E_ARG, "the call's pass and task tables need " + std::to_string(bytes) +
+                                        " bytes, more than the " + std::to_string(STATIC_SCRATCH_BYTES) +
+                                        " of cellato_hip_multi_static's scratch: split the grids over several calls "
+                                        "or use cellato_hip_multi");
+    for (int i = 0; i < n; ++i) result_in_b[i] = S.result_in_b[i];
+    if (S.tasks.empty()) return CELLATO_OK;
+    void* base = nullptr;
+    const int src = static_scratch(&base);
+    if (src != CELLATO_OK) return src;
+    // host image of the tables; the copy from pageable memory is staged before the call returns
+    std::vector<unsigned char> image(bytes);
+    std::memcpy(image.data(), S.passes.data(), S.passes.size() * sizeof(pass_args));
+    std::memcpy(image.data() + pbytes, S.tasks.data(), S.tasks.size() * sizeof(batch_task));
+    BCHECK(hipMemcpyAsync(base, image.data(), bytes, hipMemcpyHostToDevice, stream));
+    const auto* dpass = static_cast<const pass_args*>(base);
+    const auto* dtask = reinterpret_cast<const batch_task*>(static_cast<unsigned char*>(base) + pbytes);
+    // every launch on the caller's stream, rounds in order (no side streams or events to create)
+    for (auto& round : S.rounds)
+        for (const launch_rec& L : round) {
+            batch_args args{dpass, dtask + L.task_begin, L.task_count};
+            void* kargs[] = {&args};
+            BCHECK(hipLaunchKernel(L.fn, dim3(L.blocks), dim3(BLOCK_THREADS), kargs, 0, stream));
+        }
+    return CELLATO_OK;
+}
+
 } // namespace
 } // namespace cellato::hip
 
 using namespace cellato::hip;
 
 extern "C" {
+
+int cellato_hip_multi_static(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b) {
+    if ((n > 0 && (!grids || !result_in_b)) || n < 0) return bfail(CELLATO_E_ARG, "null argument");
+    return run_multi_static(grids, n, max_waves > 0 ? max_waves : 8192, static_cast<hipStream_t>(stream), result_in_b);
+}
 
 int cellato_batch_create(const cellato_batch_grid* grids, int n, int max_waves, void** handle) {
     if (!handle || (n > 0 && !grids) || n < 0) return bfail(CELLATO_E_ARG, "null argument");

@@ -129,6 +129,19 @@ int cellato_batch_plan(const cellato_batch_grid* grids, int n, int max_waves, in
  * pinned staging buffers used alternately (kept for the life of the thread). */
 int cellato_hip_multi(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b);
 
+/* cellato_hip_multi for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * Same grids, same schedule and kernels, same results; the call allocates no device or pinned
+ * memory and creates no streams or events: the pass and task tables go to a static device
+ * array of CELLATO_MULTI_STATIC_SCRATCH_BYTES (8 MiB: the tables take 144 bytes per pass and 8
+ * per task, e.g. 1000 passes of 1000 tasks) that each GPU holds for the life of the process,
+ * and every launch goes to `stream` in round order. A call whose tables do not fit is refused
+ * with CELLATO_E_ARG (nothing is launched). Every call on one GPU shares that array, so calls
+ * that overlap on a GPU (from other streams or threads) race; calls on one stream are ordered
+ * by it. The host tables are copied from pageable memory, staged before the call returns. */
+#define CELLATO_MULTI_STATIC_SCRATCH_BYTES (8u << 20)
+int cellato_hip_multi_static(const cellato_batch_grid* grids, int n, int max_waves, void* stream, int* result_in_b);
+
 /* ---- conversions on the device -------------------------------------------------------------
  * values: device array of width*height int32 state values (enum values, row-major), the
  * reference's standard grid. Asynchronous on `stream`.

@@ -99,3 +99,18 @@ def test_plan_rejects_what_the_traversers_reject():
         with pytest.raises(_lib.CellatoError) as e:
             _lib.plan(*args)
         assert e.value.code == code
+
+
+def test_multi_static_refuses_tables_beyond_its_scratch():
+    """The allocation-free batch call checks its table size on the host before touching the
+    GPU: 4M one-row tasks (32 MiB of task table) exceed its 8 MiB static array."""
+    L = _lib.lib()
+    arr = (_lib.BatchGrid * 1)()
+    g = arr[0]
+    g.rule, g.layout, g.width, g.height, g.steps, g.t0, g.temporal_steps, g.chunk_rows = 0, 0, 4096, 1 << 22, 1, 0, 1, 1
+    g.a[0], g.b[0] = FAKE[0], FAKE[1]
+    in_b = (ctypes.c_int * 1)()
+    L.cellato_hip_multi_static.argtypes = L.cellato_hip_multi.argtypes
+    rc = L.cellato_hip_multi_static(arr, 1, 0, None, in_b)
+    assert rc == _lib.E_ARG
+    assert b"cellato_hip_multi_static's scratch" in L.cellato_hip_last_error()
