@@ -253,17 +253,53 @@ __device__ __forceinline__ word tiled_right(word own, word right) {
                 __builtin_amdgcn_bitop3_b32(hi32(own) >> 1, hi32(right) << 7, 0x7F7F7F7Fu, 0xE4));
 }
 
-// Tiled layout: a word is an 8x8 block; cell (x, y) at bit (y % 8) * 8 + x % 8.
-template <int P, int X, bool HALO, class Targets = klist<>>
+// The edge word of a tiled block: column 0 holds column 7 of the block to the left, column 7
+// holds column 0 of the block to the right (the other columns are unused). Formed once when a
+// block enters a ring slot, from the neighbouring lanes' words; every later read of a
+// horizontal neighbour combines the block with it (three-input select c ? a : b per half).
+__device__ __forceinline__ word tiled_edges(word left, word right) {
+    return join(__builtin_amdgcn_bitop3_b32(lo32(left) >> 7, lo32(right) << 7, 0x01010101u, 0xE4),
+                __builtin_amdgcn_bitop3_b32(hi32(left) >> 7, hi32(right) << 7, 0x01010101u, 0xE4));
+}
+// the cell to the left / right of every cell of a block, from the block and its edge word
+__device__ __forceinline__ word tiled_left_e(word own, word e) {
+    return join(__builtin_amdgcn_bitop3_b32(lo32(own) << 1, lo32(e), 0xFEFEFEFEu, 0xE4),
+                __builtin_amdgcn_bitop3_b32(hi32(own) << 1, hi32(e), 0xFEFEFEFEu, 0xE4));
+}
+__device__ __forceinline__ word tiled_right_e(word own, word e) {
+    return join(__builtin_amdgcn_bitop3_b32(lo32(own) >> 1, lo32(e), 0x7F7F7F7Fu, 0xE4),
+                __builtin_amdgcn_bitop3_b32(hi32(own) >> 1, hi32(e), 0x7F7F7F7Fu, 0xE4));
+}
+
+// Whether the tiled ring slots of a rule carry edge words (formed once per block and stage)
+// or the kernel exchanges neighbouring words between lanes where a horizontal neighbour is
+// read. Edge words pay for rules that read neighbours beyond the Moore counts of fixed states
+// (those come from row sums, which exchange one mask per counted state): Margolus blocks,
+// direct neighbour reads, von Neumann counts, counts with a per-cell target. They cost 2 VGPRs
+// per plane and ring slot, so only while P x T <= 10. Measured at 16384^2 per launch: Critters
+// T=4 45.4 -> 40.0 us, T=8 96.2 -> 83.3; Cyclic T=2 133.6 -> 116.7 but T=4 241.8 -> 344.9;
+// Game of Life and WireWorld slower with them at every depth.
+template <class Eval>
+constexpr bool tiled_edge_words(int T) {
+    using namespace cellato::evaluators::bit_planes;
+    constexpr unsigned reads = TRAIT_VON_NEUMANN | TRAIT_MARGOLUS | TRAIT_NEIGHBOR | TRAIT_DYNAMIC_TARGET;
+    return (rule_traits<typename Eval::expression>::value & reads) != 0 && Eval::planes * T <= 10;
+}
+
+// Tiled layout: a word is an 8x8 block; cell (x, y) at bit (y % 8) * 8 + x % 8. Each ring slot
+// holds the block's P plane words and, with EDGES, their P edge words; without, horizontal
+// neighbours come from the neighbouring lanes where they are read (`halo`: the block outside
+// the segment end, for end lanes that load it).
+template <int P, class Targets, int X, bool HALO, bool EDGES>
 struct tiled_window {
     static constexpr int NS = Targets::size;
     static constexpr bool row_sums = NS > 0;
     const row_t<P>& top;
     const row_t<P>& mid;
     const row_t<P>& bot;
-    const row_t<P>& htop;
-    const row_t<P>& hmid;
-    const row_t<P>& hbot;
+    const row_t<P>& etop;   // edge words (EDGES) or the halo column's words (HALO)
+    const row_t<P>& emid;
+    const row_t<P>& ebot;
     xch<X> x;
     int t;
     const tsums<NS>& stop;
@@ -298,38 +334,43 @@ struct tiled_window {
         else return (m_[p] >> 8) | (b_[p] << 56);
     }
 
+    // e: the vertically shifted edge word (EDGES) or halo word (HALO) of the same plane
     template <int dx>
-    __device__ word hshift(word own, word halo) const {
+    __device__ word hshift(word own, word e) const {
         if constexpr (dx == 0) {
             return own;
+        } else if constexpr (EDGES) {
+            return dx < 0 ? tiled_left_e(own, e) : tiled_right_e(own, e);
         } else if constexpr (dx < 0) {
-            const word l = join(x.left(lo32(own), HALO ? lo32(halo) : 0u), x.left(hi32(own), HALO ? hi32(halo) : 0u));
-            return tiled_left(own, l);
+            return tiled_left(own, join(x.left(lo32(own), HALO ? lo32(e) : 0u), x.left(hi32(own), HALO ? hi32(e) : 0u)));
         } else {
-            const word r = join(x.right(lo32(own), HALO ? lo32(halo) : 0u), x.right(hi32(own), HALO ? hi32(halo) : 0u));
-            return tiled_right(own, r);
+            return tiled_right(own, join(x.right(lo32(own), HALO ? lo32(e) : 0u), x.right(hi32(own), HALO ? hi32(e) : 0u)));
         }
     }
+
+    static constexpr bool SIDE = EDGES || HALO;   // the second set of slot words is read
 
     template <int dx, int dy>
     __device__ vint<P> nb() const {
         vint<P> r;
 #pragma unroll
         for (int p = 0; p < P; ++p)
-            r.p[p] = hshift<dx>(vshift<dy>(top, mid, bot, p), HALO ? vshift<dy>(htop, hmid, hbot, p) : word(0));
+            r.p[p] = hshift<dx>(vshift<dy>(top, mid, bot, p),
+                                dx != 0 && SIDE ? vshift<dy>(etop, emid, ebot, p) : word(0));
         return r;
     }
 
+    // F is per cell, so F of the edge (halo) words gives F of the neighbouring blocks' cells
     template <int dx, int dy, class F>
     __device__ word nbf(F f) const {
-        vint<P> own, halo;
+        vint<P> own, side;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             own.p[p] = vshift<dy>(top, mid, bot, p);
-            halo.p[p] = HALO ? vshift<dy>(htop, hmid, hbot, p) : word(0);
+            side.p[p] = dx != 0 && SIDE ? vshift<dy>(etop, emid, ebot, p) : word(0);
         }
         if constexpr (dx == 0) return f(own);
-        else return hshift<dx>(f(own), HALO ? f(halo) : word(0));
+        else return hshift<dx>(f(own), SIDE ? f(side) : word(0));
     }
 };
 
@@ -356,19 +397,29 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const word* p, uint
 __device__ __forceinline__ word buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(word, __builtin_amdgcn_raw_buffer_load_b64(r, static_cast<int>(voff), static_cast<int>(soff), 0));
 }
+// AUX: cache-policy bits of the store (gfx9.4+: 1 sc0, 2 nt, 16 sc1)
+template <int AUX = 0>
 __device__ __forceinline__ void buf_store(word v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
-                                          r, static_cast<int>(voff), static_cast<int>(soff), 0);
+                                          r, static_cast<int>(voff), static_cast<int>(soff), AUX);
 }
 
 // Rows per unrolled loop body: a multiple of 3 so that the 3-row windows rotate by renaming
 // registers instead of moving them.
-constexpr int unroll_rows(int T) { return T <= 2 ? 6 : 3; }
+constexpr int unroll_rows(int T) { return T == 1 ? 12 : T == 2 ? 6 : 3; }
 // Rows loaded ahead of use. At the memory-bound depths a loaded row then lives in one register
-// for exactly unroll_rows steps (in flight, then in the 3-row window), so registers rotate by
-// renaming with no copies; a copy would make the wait for its load come early. (Measured at
-// 16384^2: 6 rows per body, 3 ahead beats 9 per body, 6 ahead, for every rule but Critters.)
-constexpr int prefetch_rows(int T) { return T <= 2 ? unroll_rows(T) - 3 : unroll_rows(T); }
+// for unroll_rows - prefetch_rows steps in flight and 3 in the window, so registers rotate by
+// renaming with no copies; a copy would make the wait for its load come early. Measured at
+// 16384^2: for one generation, 12 rows per body with 6 ahead beats 6 / 3 (Game of Life 12.2 ->
+// 11.9 us per launch); at depth 2, 6 / 3 beats 9 / 6 for every rule but Critters.
+constexpr int prefetch_rows(int T) { return T == 1 ? 6 : T == 2 ? 3 : unroll_rows(T); }
+// Cache policy of the stores (gfx9.4+ bits: 1 sc0, 2 nt, 16 sc1). sc1 = device scope: the line
+// is written through the XCD's L2, so that no dirty lines wait for the end-of-kernel write-back
+// that makes them visible to the other XCDs' next launch. Measured at 16384^2 per launch: one
+// generation, Game of Life 12.2 -> 11.6 us, WireWorld 21.7 -> 21.4 us; Game of Life T=4 / T=8
+// 6 % faster, Critters 2-7 %, WireWorld 1-2 % (T=2 3 % slower), HPP within 2 %. nt (streaming)
+// stores bypass the Infinity Cache that the next launch reads from: 30-50 % slower.
+constexpr int store_aux(int) { return 16; }
 
 // Row sums of a linear ring slot: for each counted state K (dictionary index), the mask of
 // the slot's row and its two 1-cell shifts, added as a full adder. Called once per row and
@@ -398,40 +449,48 @@ struct row_sum_fill<klist<K...>> {
     }
 };
 
-// Row sums of a tiled ring slot: the counted state's mask of the block and of its left and
-// right neighbour blocks (two lane exchanges of one word per state), shifted by one cell
-// within each 8-cell block row, added as a full adder. `halo` = the block outside the
-// segment end for end lanes that load it (one generation per launch), else unused.
+// Row sums of a tiled ring slot: the counted state's mask of the block and of the neighbouring
+// blocks' edge cells, shifted by one cell within each 8-cell block row, added as a full adder.
+// The neighbours' masks come from the slot's edge words (EDGES), or from the neighbouring
+// lanes (two lane exchanges of one word per state; `side` = the block outside the segment end
+// for end lanes that load it, HALO).
 template <class Targets>
 struct tiled_sum_fill;
 template <long long... K>
 struct tiled_sum_fill<klist<K...>> {
-    template <bool LOADS_HALO, int P, int NS, class XC>
-    __device__ __forceinline__ static void run(const word (&w)[P], const word (&halo)[P], const XC& xc, tsums<NS>& out) {
-        (one<K, kindex<K, klist<K...>>::value, LOADS_HALO>(w, halo, xc, out), ...);
+    template <bool EDGES, bool HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void run(const word (&w)[P], const word (&side)[P], const XC& xc, tsums<NS>& out) {
+        (one<K, kindex<K, klist<K...>>::value, EDGES, HALO>(w, side, xc, out), ...);
     }
-    template <long long Kj, int j, bool LOADS_HALO, int P, int NS, class XC>
-    __device__ __forceinline__ static void one(const word (&w)[P], const word (&halo)[P], const XC& xc, tsums<NS>& out) {
+    template <long long Kj, int j, bool EDGES, bool HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void one(const word (&w)[P], const word (&side)[P], const XC& xc, tsums<NS>& out) {
         using namespace cellato::evaluators::bit_planes;
         const is_index<Kj> f{};
-        vint<P> own, hv;
+        vint<P> own, sv;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             own.p[p] = w[p];
-            hv.p[p] = halo[p];
+            sv.p[p] = side[p];
         }
         const word m = f(own);
-        word ml, mr;   // the mask of the block to the left / right
-        if constexpr (LOADS_HALO) {
-            const word mh = f(hv);
-            ml = join(xc.left(lo32(m), lo32(mh)), xc.left(hi32(m), hi32(mh)));
-            mr = join(xc.right(lo32(m), lo32(mh)), xc.right(hi32(m), hi32(mh)));
+        word at_left, at_right;   // cells x - 1 and x + 1
+        if constexpr (EDGES) {
+            const word me = f(sv);
+            at_left = tiled_left_e(m, me);
+            at_right = tiled_right_e(m, me);
         } else {
-            ml = join(xc.left0(lo32(m)), xc.left0(hi32(m)));
-            mr = join(xc.right0(lo32(m)), xc.right0(hi32(m)));
+            word ml, mr;   // the mask of the block to the left / right
+            if constexpr (HALO) {
+                const word mh = f(sv);
+                ml = join(xc.left(lo32(m), lo32(mh)), xc.left(hi32(m), hi32(mh)));
+                mr = join(xc.right(lo32(m), lo32(mh)), xc.right(hi32(m), hi32(mh)));
+            } else {
+                ml = join(xc.left0(lo32(m)), xc.left0(hi32(m)));
+                mr = join(xc.right0(lo32(m)), xc.right0(hi32(m)));
+            }
+            at_left = tiled_left(m, ml);
+            at_right = tiled_right(m, mr);
         }
-        const word at_left = tiled_left(m, ml);     // cell x - 1
-        const word at_right = tiled_right(m, mr);   // cell x + 1
         fa(at_left, m, at_right, out.s[j], out.c[j]);
     }
 };
@@ -440,14 +499,18 @@ struct tiled_sum_fill<klist<K...>> {
 // TPAR: -1, or the parity of a.t0 known at compile time; the windows then carry only the
 // parity of each stage's step, and a rule that reads it (alternation, Margolus blocks) keeps
 // one of its two forms.
-template <class Eval, int LAYOUT, int T, int X, int HALO, int TPAR = -1>
+// UR / PF: rows per unrolled loop body and rows loaded ahead (PF <= UR); SAUX: cache-policy
+// bits of the stores.
+template <class Eval, int LAYOUT, int T, int X, int HALO, int TPAR = -1, int UR = unroll_rows(T), int PF = prefetch_rows(T),
+          int SAUX = store_aux(T)>
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     constexpr int P = Eval::planes;
     constexpr bool LOADS_HALO = HALO == HALO_LOAD;
     // lane groups walk different rows: rows narrower than a wave, and the halo-lane tail
     constexpr bool GROUPED = X == XCH_PERMUTE || HALO == HALO_LANES;
-    constexpr int U = unroll_rows(T);
-    constexpr int D = prefetch_rows(T);
+    constexpr int U = UR;
+    constexpr int D = PF;
+    static_assert(UR % 3 == 0 && PF >= 1 && PF <= UR, "rows per body: a multiple of 3; prefetch at most one body");
     static_assert(!(HALO == HALO_LOAD && T != 1), "loaded halos are exact for one generation only");
 
     const int lane = lane_id();
@@ -510,9 +573,10 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     // edge halves; tiled rows exchange whole words when used)
     using targets = typename Eval::row_sum_targets;
     constexpr int NS = targets::size;
+    constexpr bool EDGES = LAYOUT == TILED && tiled_edge_words<Eval>(T);
     lrow<P, NS> lwin[T][3];
     word win[T][3][P];
-    word hwin[3][P];       // halo column, generation 0 (HALO_LOAD, tiled)
+    word wedge[T][3][P];   // tiled ring slots: edge words (EDGES), or the halo column of stage 0
     tsums<NS> tsum[T][3];  // row sums of the tiled ring slots
     word pf[U][P];         // prefetched generation-0 rows
     word hpf[U][P];
@@ -523,6 +587,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 win[k][i][p] = 0;
+                wedge[k][i][p] = 0;
                 lwin[k][i].w[p] = 0;
                 lwin[k][i].l[p] = 0;
                 lwin[k][i].r[p] = 0;
@@ -537,10 +602,6 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 lwin[k][i].rs_c[j] = 0;
             }
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int p = 0; p < P; ++p) hwin[i][p] = 0;
-#pragma unroll
     for (int k = 0; k < T; ++k)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
@@ -550,27 +611,34 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 tsum[k][i].c[j] = 0;
             }
 
-    // row of the next load, kept wrapped onto [0, h) incrementally (no division in the loop)
+    // Step i loads row s = r0 - T + i; every group runs the same number of steps, so the
+    // loop is uniform across the wave even when groups walk different rows.
+    const int nsteps = a.chunk + 2 * T;
+
+    // row of the next load, kept wrapped onto [0, h) incrementally (no division in the loop).
+    // The prefetch of the last body reaches past the task's rows (steps >= nsteps): those loads
+    // go to the offset past the plane (they read 0 and move no data); no step uses them.
     int lrow_next = wrap(r0 - T, a.h);
+    int lstep = 0;
     auto load_row = [&](word (&dst)[P], word (&hdst)[P]) {
+        const bool live = lstep < nsteps;   // uniform across the wave
         const uint32_t roff = static_cast<uint32_t>(lrow_next) * row_bytes;
+        const uint32_t coff = live ? col_off : OOB_OFFSET;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            if constexpr (GROUPED) dst[p] = buf_load(rin[p], roff + col_off, 0u);
-            else dst[p] = buf_load(rin[p], col_off, roff);
+            if constexpr (GROUPED) dst[p] = buf_load(rin[p], live ? roff + col_off : OOB_OFFSET, 0u);
+            else dst[p] = buf_load(rin[p], coff, roff);
             hdst[p] = 0;
         }
         if constexpr (LOADS_HALO) {
-            const uint32_t hoff = __builtin_elementwise_min(__builtin_elementwise_add_sat(roff, halo_base), OOB_OFFSET);
+            const uint32_t hoff = live ? __builtin_elementwise_min(__builtin_elementwise_add_sat(roff, halo_base), OOB_OFFSET)
+                                       : OOB_OFFSET;
 #pragma unroll
             for (int p = 0; p < P; ++p) hdst[p] = buf_load(rin[p], hoff, 0u);
         }
         lrow_next = lrow_next + 1 == a.h ? 0 : lrow_next + 1;
+        ++lstep;
     };
-
-    // Step i loads row s = r0 - T + i; every group runs the same number of steps, so the
-    // loop is uniform across the wave even when groups walk different rows.
-    const int nsteps = a.chunk + 2 * T;
 #pragma unroll
     for (int u = 0; u < D; ++u) load_row(pf[u], hpf[u]);
     int ycur = wrap(r0 - T - 2, a.h);   // advanced to row s - 1 (the row stage 0 computes) at each step
@@ -600,13 +668,27 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                         lwin[0][nw].r[p] = xc.right0(lo32(v));
                     }
                 } else {
-                    win[0][nw][p] = pf[u][p];
-                    if constexpr (LOADS_HALO) hwin[nw][p] = hpf[u][p];
+                    const word v = pf[u][p];
+                    win[0][nw][p] = v;
+                    if constexpr (EDGES) {
+                        word l, r;   // the blocks to the left and right (the outside block for end lanes)
+                        if constexpr (LOADS_HALO) {
+                            const word hv = hpf[u][p];
+                            l = join(xc.left(lo32(v), lo32(hv)), xc.left(hi32(v), hi32(hv)));
+                            r = join(xc.right(lo32(v), lo32(hv)), xc.right(hi32(v), hi32(hv)));
+                        } else {
+                            l = join(xc.left0(lo32(v)), xc.left0(hi32(v)));
+                            r = join(xc.right0(lo32(v)), xc.right0(hi32(v)));
+                        }
+                        wedge[0][nw][p] = tiled_edges(l, r);
+                    } else if constexpr (LOADS_HALO) {
+                        wedge[0][nw][p] = hpf[u][p];
+                    }
                 }
             }
             if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
             if constexpr (LAYOUT == TILED && NS > 0)
-                tiled_sum_fill<targets>::template run<LOADS_HALO>(win[0][nw], hwin[nw], xc, tsum[0][nw]);
+                tiled_sum_fill<targets>::template run<EDGES, LOADS_HALO>(win[0][nw], wedge[0][nw], xc, tsum[0][nw]);
             // the load stays after this step's use of the window (hoisted above it, it would
             // overlap the register it refills with a live row and force a copy)
             if constexpr (T <= 2) __builtin_amdgcn_sched_barrier(0);
@@ -630,9 +712,11 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
                     out = Eval::evaluate(wnd);
                 } else {
-                    const tiled_window<P, X, LOADS_HALO, targets> wnd{win[k][tp], win[k][md], win[k][nw],
-                                                                      hwin[tp], hwin[md], hwin[nw], xc, t,
-                                                                      tsum[k][tp], tsum[k][md], tsum[k][nw]};
+                    // without edge words, the halo column (HALO_LOAD) is kept in stage 0's slots only
+                    const int ks = EDGES ? k : 0;
+                    const tiled_window<P, targets, X, LOADS_HALO, EDGES> wnd{
+                        win[k][tp], win[k][md], win[k][nw], wedge[ks][tp], wedge[ks][md], wedge[ks][nw], xc, t,
+                        tsum[k][tp], tsum[k][md], tsum[k][nw]};
                     out = Eval::evaluate(wnd);
                 }
                 if (k + 1 < T) {
@@ -644,12 +728,16 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                             lwin[kn][nw].l[p] = xc.left0(hi32(out.p[p]));
                             lwin[kn][nw].r[p] = xc.right0(lo32(out.p[p]));
                         } else {
-                            win[kn][nw][p] = out.p[p];
+                            const word v = out.p[p];
+                            win[kn][nw][p] = v;
+                            if constexpr (EDGES)
+                                wedge[kn][nw][p] = tiled_edges(join(xc.left0(lo32(v)), xc.left0(hi32(v))),
+                                                               join(xc.right0(lo32(v)), xc.right0(hi32(v))));
                         }
                     }
                     if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
                     if constexpr (LAYOUT == TILED && NS > 0)
-                        tiled_sum_fill<targets>::template run<false>(win[kn][nw], hwin[nw], xc, tsum[kn][nw]);
+                        tiled_sum_fill<targets>::template run<EDGES, false>(win[kn][nw], wedge[kn][nw], xc, tsum[kn][nw]);
                 }
             }
             // generation T of row s - T (in [r0, r1) implies in [0, h))
@@ -657,7 +745,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             const bool stored = (yo >= r0) & (yo < r1) & owner;   // no short circuit (see OOB_OFFSET)
             const uint32_t soff = stored ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
 #pragma unroll
-            for (int p = 0; p < P; ++p) buf_store(out.p[p], rout[p], soff, 0u);
+            for (int p = 0; p < P; ++p) buf_store<SAUX>(out.p[p], rout[p], soff, 0u);
         }
     };
     // The fill runs in a copy of the loop that tests each stage, the steady state in one
@@ -679,7 +767,7 @@ __device__ __forceinline__ int xcd_block(int b, int nblocks) {
 // One task of a pass whose rows are cut into 62-column halo-lane segments plus a tail of
 // narrow lane groups. Both kinds run the same loop (one copy of the code): only the lane
 // mapping differs, and it is set up here, uniformly across the wave.
-template <class Eval, int LAYOUT, int T, int TPAR = -1>
+template <class Eval, int LAYOUT, int T, int TPAR = -1, int SAUX = store_aux(T)>
 __device__ __forceinline__ void segmented_task(const pass_args& a, int task) {
     pass_args b = a;
     int t = task;
@@ -691,10 +779,10 @@ __device__ __forceinline__ void segmented_task(const pass_args& a, int task) {
         b.nseg = 1;
         t = task - a.main_tasks;
     }
-    stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES, TPAR>(b, t);
+    stream_task<Eval, LAYOUT, T, XCH_SEGMENT, HALO_LANES, TPAR, unroll_rows(T), prefetch_rows(T), SAUX>(b, t);
 }
 
-template <class Eval, int LAYOUT, int T, int X, int HALO>
+template <class Eval, int LAYOUT, int T, int X, int HALO, int UR = unroll_rows(T), int PF = prefetch_rows(T), int SAUX = store_aux(T)>
 __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
     const int blk = xcd_block(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int task = __builtin_amdgcn_readfirstlane(blk * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / 64);
@@ -703,16 +791,16 @@ __global__ void __launch_bounds__(BLOCK_THREADS) stream_kernel(pass_args a) {
         // one copy of the task per parity of the first step (a uniform branch: a launch
         // runs one of them)
         if (a.t0 & 1) {
-            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 1>(a, task);
-            else stream_task<Eval, LAYOUT, T, X, HALO, 1>(a, task);
+            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 1, SAUX>(a, task);
+            else stream_task<Eval, LAYOUT, T, X, HALO, 1, UR, PF, SAUX>(a, task);
         } else {
-            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 0>(a, task);
-            else stream_task<Eval, LAYOUT, T, X, HALO, 0>(a, task);
+            if constexpr (HALO == HALO_LANES) segmented_task<Eval, LAYOUT, T, 0, SAUX>(a, task);
+            else stream_task<Eval, LAYOUT, T, X, HALO, 0, UR, PF, SAUX>(a, task);
         }
     } else if constexpr (HALO == HALO_LANES) {
-        segmented_task<Eval, LAYOUT, T>(a, task);
+        segmented_task<Eval, LAYOUT, T, -1, SAUX>(a, task);
     } else {
-        stream_task<Eval, LAYOUT, T, X, HALO>(a, task);
+        stream_task<Eval, LAYOUT, T, X, HALO, -1, UR, PF, SAUX>(a, task);
     }
 }
 
@@ -728,14 +816,14 @@ struct batch_args {
     int ntasks;
 };
 
-template <class Eval, int LAYOUT, int T, int TPAR>
+template <class Eval, int LAYOUT, int T, int TPAR, int SAUX = store_aux(T)>
 __device__ __forceinline__ void batch_task_run(const pass_args& a, int task) {
     if (a.variant == VARIANT_SEGMENT)
-        segmented_task<Eval, LAYOUT, T, TPAR>(a, task);
+        segmented_task<Eval, LAYOUT, T, TPAR, SAUX>(a, task);
     else if (a.variant == VARIANT_RING64)
-        stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE, TPAR>(a, task);
+        stream_task<Eval, LAYOUT, T, XCH_RING64, HALO_NONE, TPAR, unroll_rows(T), prefetch_rows(T), SAUX>(a, task);
     else
-        stream_task<Eval, LAYOUT, T, XCH_PERMUTE, HALO_NONE, TPAR>(a, task);
+        stream_task<Eval, LAYOUT, T, XCH_PERMUTE, HALO_NONE, TPAR, unroll_rows(T), prefetch_rows(T), SAUX>(a, task);
 }
 
 // A fixed grid of waves walks a task list that mixes passes of different grids (sizes, lane

@@ -142,4 +142,4 @@ def test_no_reachable_kernel_uses_scratch(rule):
         eff, priv, regs = _lib.kernel_info(rid, 0, 4096, T, batch=True)
         assert priv == 0 and 1 <= eff <= T, (T, eff, priv, regs)
     if rule == "cyclic":
-        assert _lib.kernel_info(rid, 0, 16384, 8)[0] == 4
+        assert _lib.kernel_info(rid, 0, 16384, 8)[0] == 4 and _lib.kernel_info(rid, 1, 16384, 8)[0] == 8
