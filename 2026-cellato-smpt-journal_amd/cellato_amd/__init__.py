@@ -3,8 +3,9 @@ include/cellato_hip.h (libcellato_hip.so)."""
 from . import _lib, grid, rules
 from ._lib import CellatoError, LAYOUT_LINEAR, LAYOUT_TILED
 from .rules import Rule, all_rules, rule, iid_values
-from .planes import PlaneGrid, DeviceGrid
-from .traversers import RunParams, SimpleTraverser, TemporalTraverser
+from .planes import BitArrayGrid, PlaneGrid, DeviceGrid
+from .traversers import BitArrayTraverser, RunParams, SimpleTraverser, TemporalTraverser
 
 __all__ = ["CellatoError", "LAYOUT_LINEAR", "LAYOUT_TILED", "Rule", "all_rules", "rule", "iid_values",
-           "PlaneGrid", "DeviceGrid", "RunParams", "SimpleTraverser", "TemporalTraverser", "grid", "rules"]
+           "BitArrayGrid", "PlaneGrid", "DeviceGrid", "BitArrayTraverser", "RunParams", "SimpleTraverser",
+           "TemporalTraverser", "grid", "rules"]

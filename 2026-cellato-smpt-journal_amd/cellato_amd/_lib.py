@@ -91,6 +91,8 @@ def lib():
         L.cellato_batch_destroy.argtypes = [ctypes.c_void_p]
         L.cellato_hip_plan.argtypes = [ctypes.c_int] * 6 + [ctypes.POINTER(ctypes.c_int)] * 3
         L.cellato_hip_kernel_info.argtypes = [ctypes.c_int] * 5 + [ctypes.POINTER(ctypes.c_int)] * 3
+        L.cellato_hip_bit_array.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p] + \
+            [ctypes.c_int] * 4 + [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.cellato_batch_plan.argtypes = [ctypes.POINTER(BatchGrid), ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong)]
@@ -143,4 +145,12 @@ def temporal(rule, layout, ptrs_a, ptrs_b, width, height, t0, steps, temporal_st
     check(lib().cellato_hip_temporal(rule, layout, plane_array(ptrs_a), plane_array(ptrs_b), width, height, t0,
                                      steps, temporal_steps, chunk_rows, ctypes.c_void_p(stream or 0),
                                      ctypes.byref(res)))
+    return bool(res.value)
+
+
+def bit_array(rule, word_bits, ptr_a, ptr_b, width, height, t0, steps, stream=None):
+    """cellato_hip_bit_array (one device array of words per buffer); True when the result is in b."""
+    res = ctypes.c_int(-1)
+    check(lib().cellato_hip_bit_array(rule, word_bits, ctypes.c_void_p(int(ptr_a)), ctypes.c_void_p(int(ptr_b)), width,
+                                      height, t0, steps, ctypes.c_void_p(stream or 0), ctypes.byref(res)))
     return bool(res.value)

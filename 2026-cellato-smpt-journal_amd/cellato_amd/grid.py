@@ -1,7 +1,7 @@
 """Host-side grids: packing of dictionary indices into 64-bit bit planes (linear and tiled
 layouts, as include/memory/bit_planes_grid.hpp:234-270 and tiled_bit_planes_grid.hpp:243-288
-define them), unpacking, and the reference's 16-tile checksum
-(include/memory/standard_grid.hpp:179-213)."""
+define them) and into bit_array words (include/memory/bit_array_grid.hpp:293-312), unpacking,
+and the reference's 16-tile checksum (include/memory/standard_grid.hpp:179-213)."""
 import numpy as np
 
 LINEAR, TILED = 0, 1
@@ -47,6 +47,29 @@ def unpack(planes_arr, width, height, layout=LINEAR):
             cells = bits.reshape(rows, words, 8, 8).transpose(0, 2, 1, 3).reshape(height, width)
         idx |= cells.astype(np.int32) << b
     return idx
+
+
+def pack_bit_array(indices, planes, word_bits=64):
+    """indices: (H, W) dictionary indices -> (H, W / cpw) words (uint32 or uint64), cpw =
+    word_bits // planes cells per word, cell x of a row at bits [(x % cpw) * P, + P) of word
+    x // cpw (rows start words: W must be a multiple of cpw)."""
+    idx = np.ascontiguousarray(indices)
+    H, W = idx.shape
+    cpw = word_bits // planes
+    if W % cpw:
+        raise ValueError(f"Grid x_size must be divisible by cells_per_word. x_size={W}, cells_per_word={cpw}")
+    dt = np.uint64 if word_bits == 64 else np.uint32
+    cells = idx.reshape(H, W // cpw, cpw).astype(np.uint64) & np.uint64((1 << planes) - 1)
+    shifts = (np.arange(cpw, dtype=np.uint64) * np.uint64(planes))
+    return np.bitwise_or.reduce(cells << shifts, axis=-1).astype(dt)
+
+
+def unpack_bit_array(words, width, height, planes, word_bits=64):
+    """(H, W / cpw) words -> (H, W) dictionary indices (int32)."""
+    cpw = word_bits // planes
+    w = np.ascontiguousarray(words).astype(np.uint64).reshape(height, width // cpw, 1)
+    shifts = (np.arange(cpw, dtype=np.uint64) * np.uint64(planes))
+    return ((w >> shifts) & np.uint64((1 << planes) - 1)).astype(np.int32).reshape(height, width)
 
 
 def checksum(values):

@@ -110,3 +110,37 @@ class DeviceGrid:
         _lib.check(_lib.lib().cellato_hip_checksum(self.rule.id, self.layout, _lib.plane_array(self.ptrs(self.cur)),
                                                    self.width, self.height, out.data_ptr(), self._stream()))
         return [int(x) for x in out.cpu().tolist()]
+
+
+class BitArrayGrid:
+    """The reference's bit_array::grid (include/memory/bit_array_grid.hpp:85-321): every cell's
+    dictionary index in P bits, word_bits // P cells per word (32- or 64-bit words), row-major."""
+
+    def __init__(self, rule, width, height, words, word_bits=32):
+        self.rule = get_rule(rule) if not hasattr(rule, "planes") else rule
+        self.width, self.height, self.word_bits = width, height, word_bits
+        self.words = words   # (H, W / cells_per_word) uint32 / uint64
+
+    @property
+    def cells_per_word(self):
+        return self.word_bits // self.rule.planes
+
+    @classmethod
+    def from_standard(cls, rule, values, word_bits=32):
+        r = get_rule(rule) if not hasattr(rule, "planes") else rule
+        v = np.asarray(values)
+        H, W = v.shape
+        return cls(r, W, H, hg.pack_bit_array(r.index_of(v), r.planes, word_bits), word_bits)
+
+    def to_standard(self):
+        return self.rule.value_of(hg.unpack_bit_array(self.words, self.width, self.height, self.rule.planes,
+                                                      self.word_bits))
+
+    def x_size_physical(self):
+        return self.width // self.cells_per_word
+
+    def y_size_physical(self):
+        return self.height
+
+    def get_checksum(self):
+        return hg.checksum_str(self.to_standard())

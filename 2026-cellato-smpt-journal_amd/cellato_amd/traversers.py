@@ -6,8 +6,10 @@ state is copied to the host after every launch, the reference's VERBOSE mode), a
 """
 from dataclasses import dataclass
 
+import numpy as np
+
 from . import _lib
-from .planes import DeviceGrid, PlaneGrid
+from .planes import BitArrayGrid, DeviceGrid, PlaneGrid
 
 
 @dataclass
@@ -95,3 +97,56 @@ class TemporalTraverser(_Base):
     def _launch(self, rule, layout, a, b, w, h, t0, steps):
         return _lib.temporal(rule, layout, a, b, w, h, t0, steps, self._params.temporal_steps,
                              self._params.chunk_rows, _stream())
+
+
+class BitArrayTraverser:
+    """The simple traverser over the bit_array layout (reference: evaluators::bit_array under
+    traversers/cuda/simple.cu): one generation per launch, every cell evaluated on its own."""
+    is_CUDA = True
+
+    def __init__(self):
+        self._buf = None
+        self._t = 0
+
+    def init(self, grid: BitArrayGrid, params: RunParams):
+        import torch
+        if params.word_size not in (0, grid.word_bits):
+            raise RuntimeError("word_size does not match the grid's word type")
+        self._grid = grid
+        dt = torch.int64 if grid.word_bits == 64 else torch.int32
+        host = np.ascontiguousarray(grid.words).view(np.int64 if grid.word_bits == 64 else np.int32)
+        a = torch.from_numpy(host.copy()).to("cuda")
+        self._buf = [a, torch.zeros_like(a, dtype=dt)]
+        self._cur = 0
+        self._t = 0
+
+    def run(self, steps, callback=None):
+        if self._buf is None:
+            raise RuntimeError("run() before init()")
+        if callback is None:
+            self._advance(steps)
+            return
+        callback(0, self._host())
+        for k in range(steps):
+            self._advance(1)
+            callback(k + 1, self._host())
+
+    def fetch_result(self) -> BitArrayGrid:
+        import torch
+        torch.cuda.synchronize()
+        g = self._host()
+        self._buf = None
+        return g
+
+    def _host(self):
+        g = self._grid
+        words = self._buf[self._cur].cpu().numpy().view(np.uint64 if g.word_bits == 64 else np.uint32).copy()
+        return BitArrayGrid(g.rule, g.width, g.height, words, g.word_bits)
+
+    def _advance(self, steps):
+        g = self._grid
+        in_b = _lib.bit_array(g.rule.id, g.word_bits, self._buf[self._cur].data_ptr(),
+                              self._buf[1 - self._cur].data_ptr(), g.width, g.height, self._t, steps, _stream())
+        if in_b:
+            self._cur = 1 - self._cur
+        self._t += steps

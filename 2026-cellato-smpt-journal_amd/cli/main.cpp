@@ -3,9 +3,10 @@
 // reference's sweep scripts run unchanged; one CSV line per experiment on stdout in the
 // reference's columns (include/experiments/experiment_report.hpp:82-93), a readable report
 // on stderr. Runs the bit_planes and tiled_bit_planes evaluators with the simple and temporal
-// traversers of all ten automata on the GPU (--device HIP, or CUDA as the reference's
-// scripts spell it); other combinations (CPU paths, the standard and bit_array evaluators,
-// --reference_impl) print the reference's "No suitable ..." message.
+// traversers, and the bit_array evaluator (32- or 64-bit words) with the simple traverser, for
+// all ten automata on the GPU (--device HIP, or CUDA as the reference's scripts spell it);
+// other combinations (CPU paths, the standard evaluator, --reference_impl) print the
+// reference's "No suitable ..." message.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -16,10 +17,13 @@
 #include <vector>
 
 #include "cellato/automata/all.hpp"
+#include "cellato/evaluators/bit_array.hpp"
 #include "cellato/evaluators/bit_planes.hpp"
+#include "cellato/evaluators/tiled_bit_planes.hpp"
 #include "cellato/experiments/experiment_report.hpp"
 #include "cellato/experiments/initial_state.hpp"
 #include "cellato/experiments/run_params.hpp"
+#include "cellato/memory/bit_array_grid.hpp"
 #include "cellato/memory/bit_planes_grid.hpp"
 #include "cellato/memory/standard_grid.hpp"
 #include "cellato/traversers/hip/traversers.hpp"
@@ -58,8 +62,8 @@ void print_usage() {
                  "                                  traffic, fluid, critters, cyclic\n"
                  "  --device <name>                 HIP (CUDA is accepted as the same device)\n"
                  "  --traverser <name>              simple, temporal\n"
-                 "  --evaluator <name>              bit_planes, tiled_bit_planes\n"
-                 "  --layout <name>                 bit_planes, tiled_bit_planes (same as the evaluator)\n"
+                 "  --evaluator <name>              bit_planes, tiled_bit_planes, bit_array\n"
+                 "  --layout <name>                 bit_planes, tiled_bit_planes, bit_array (same as the evaluator)\n"
                  "  --x_size <number>               X size of the grid (cells)\n"
                  "  --y_size <number>               Y size of the grid (cells)\n"
                  "  --steps <number>                Number of generations\n"
@@ -128,17 +132,44 @@ run_params read_params(const options& o) {
 
 void hip_ok(hipError_t e, const char* what) { cellato::memory::grids::hip_check(e, what); }
 
+enum class layout_kind { linear, tiled, bit_array };
+
+// grid, evaluator and traverser of a suite, as the reference's test_suites.hpp:86-225 bind them
+template <class config, layout_kind L, bool Temporal, class Word>
+struct suite_types;
+template <class config, bool Temporal, class Word>
+struct suite_types<config, layout_kind::linear, Temporal, Word> {
+    using dict_t = typename config::state_dictionary;
+    using grid_t = cellato::memory::grids::bit_planes::grid<std::uint64_t, dict_t>;
+    using eval_t = cellato::evaluators::bit_planes::evaluator<std::uint64_t, dict_t, typename config::algorithm>;
+    using traverser_t = std::conditional_t<Temporal, cellato::traversers::hip::temporal::traverser<eval_t, grid_t>,
+                                           cellato::traversers::hip::simple::traverser<eval_t, grid_t>>;
+};
+template <class config, bool Temporal, class Word>
+struct suite_types<config, layout_kind::tiled, Temporal, Word> {
+    using dict_t = typename config::state_dictionary;
+    using grid_t = cellato::memory::grids::tiled_bit_planes::grid<std::uint64_t, dict_t>;
+    using eval_t = cellato::evaluators::tiled_bit_planes::evaluator<std::uint64_t, dict_t, typename config::algorithm>;
+    using traverser_t = std::conditional_t<Temporal, cellato::traversers::hip::temporal::traverser<eval_t, grid_t>,
+                                           cellato::traversers::hip::simple::traverser<eval_t, grid_t>>;
+};
+template <class config, bool Temporal, class Word>
+struct suite_types<config, layout_kind::bit_array, Temporal, Word> {
+    using dict_t = typename config::state_dictionary;
+    using grid_t = cellato::memory::grids::bit_array::grid<dict_t, Word>;
+    using eval_t = cellato::evaluators::bit_array::evaluator<grid_t, typename config::algorithm>;
+    using traverser_t = cellato::traversers::hip::simple::traverser<eval_t, grid_t>;
+};
+
 // one experiment: rounds of init -> run(steps) (timed with HIP events) -> fetch -> checksum
-template <int id, bool Tiled, bool Temporal>
+template <int id, layout_kind L, bool Temporal, class Word = std::uint64_t>
 cellato::run::experiment_report run_experiment(const run_params& p) {
     namespace g = cellato::memory::grids;
     using config = typename cellato::automata::by_id<id>::config;
     using cell_t = typename config::cell_state;
-    using dict_t = typename config::state_dictionary;
-    using grid_t = g::plane_grid<std::uint64_t, dict_t, Tiled>;
-    using eval_t = cellato::evaluators::bit_planes::evaluator<std::uint64_t, dict_t, typename config::algorithm>;
-    using traverser_t = std::conditional_t<Temporal, cellato::traversers::hip::temporal::traverser<eval_t, grid_t>,
-                                           cellato::traversers::hip::simple::traverser<eval_t, grid_t>>;
+    using types = suite_types<config, L, Temporal, Word>;
+    using grid_t = typename types::grid_t;
+    using traverser_t = typename types::traverser_t;
 
     const auto values = cellato::experiments::initial_state(p.automaton, p.x_size, p.y_size, p.seed);
     std::vector<cell_t> cells(values.size());
@@ -176,20 +207,27 @@ cellato::run::experiment_report run_experiment(const run_params& p) {
 template <int id>
 bool run_for(const run_params& p) {
     if (p.automaton != cellato::automata::by_id<id>::config::name) return false;
-    const bool tiled = p.evaluator == "tiled_bit_planes";
     const bool temporal = p.traverser == "temporal";
     cellato::run::experiment_report r;
-    if (tiled) r = temporal ? run_experiment<id, true, true>(p) : run_experiment<id, true, false>(p);
-    else r = temporal ? run_experiment<id, false, true>(p) : run_experiment<id, false, false>(p);
+    if (p.evaluator == "bit_array")
+        r = p.word_size == 32 ? run_experiment<id, layout_kind::bit_array, false, std::uint32_t>(p)
+                              : run_experiment<id, layout_kind::bit_array, false, std::uint64_t>(p);
+    else if (p.evaluator == "tiled_bit_planes")
+        r = temporal ? run_experiment<id, layout_kind::tiled, true>(p) : run_experiment<id, layout_kind::tiled, false>(p);
+    else
+        r = temporal ? run_experiment<id, layout_kind::linear, true>(p) : run_experiment<id, layout_kind::linear, false>(p);
     std::cout << r.csv_line() << std::endl;
     r.pretty_print(std::cerr);
     return true;
 }
 
-// the combinations the HIP traversers serve (the reference's test_suites.hpp:130-225 on_cuda
-// bit_planes / tiled_bit_planes suites with 64-bit words)
+// the combinations the HIP traversers serve (the reference's test_suites.hpp:86-225 on_cuda
+// suites: bit_planes / tiled_bit_planes with 64-bit words, simple or temporal; bit_array with
+// 32- or 64-bit words, simple)
 bool served(const run_params& p) {
     const bool gpu = p.device == "HIP" || p.device == "CUDA";
+    if (p.evaluator == "bit_array" && p.layout == "bit_array")
+        return gpu && p.traverser == "simple" && (p.word_size == 32 || p.word_size == 64);
     const bool evaluator = (p.evaluator == "bit_planes" && p.layout == "bit_planes") ||
                            (p.evaluator == "tiled_bit_planes" && p.layout == "tiled_bit_planes");
     return gpu && evaluator && (p.traverser == "simple" || p.traverser == "temporal") && p.word_size == 64;

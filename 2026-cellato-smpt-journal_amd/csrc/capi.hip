@@ -172,6 +172,41 @@ int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_s
     return CELLATO_OK;
 }
 
+int cellato_hip_bit_array(int rule, int word_bits, void* buf_a, void* buf_b, int width, int height, int t0, int steps,
+                          void* stream, int* result_in_b) {
+    if (rule < 0 || rule >= RULES) return fail(CELLATO_E_RULE, "unknown rule id " + std::to_string(rule));
+    if (word_bits != 32 && word_bits != 64)
+        return fail(CELLATO_E_ARG, "bit_array words are 32 or 64 bits, got " + std::to_string(word_bits));
+    if (!buf_a || !buf_b || !result_in_b) return fail(CELLATO_E_ARG, "null buffer or result pointer");
+    if (steps < 0) return fail(CELLATO_E_ARG, "negative step count");
+    const rule_kernels& k = kernels_of_rule(rule);
+    const int cpw = word_bits / k.planes;
+    if (width <= 0 || height <= 0 || width % cpw != 0)
+        return fail(CELLATO_E_GEOMETRY, "Grid x_size must be divisible by cells_per_word. x_size=" + std::to_string(width) +
+                                            ", cells_per_word=" + std::to_string(cpw));
+    const long long words = static_cast<long long>(width / cpw) * height;
+    const long long blocks = (words + BLOCK_THREADS - 1) / BLOCK_THREADS;
+    if (blocks > 0x7FFFFFFFLL) return fail(CELLATO_E_GEOMETRY, "grid too large for one launch");
+    bit_array_args a{nullptr, nullptr, width / cpw, height, 0, words};
+    void* cur = buf_a;
+    void* nxt = buf_b;
+    const void* fn = k.bit_array[word_bits == 64 ? 1 : 0];
+    for (int s = 0; s < steps; ++s) {
+        a.in = cur;
+        a.out = nxt;
+        a.t = t0 + s;
+        void* kargs[] = {&a};
+        const hipError_t e = hipLaunchKernel(fn, dim3(static_cast<unsigned>(blocks)), dim3(BLOCK_THREADS), kargs, 0,
+                                             static_cast<hipStream_t>(stream));
+        if (e != hipSuccess) return fail(CELLATO_E_HIP, std::string("kernel launch failed: ") + hipGetErrorString(e));
+        void* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    *result_in_b = steps % 2;
+    return CELLATO_OK;
+}
+
 int cellato_hip_kernel_info(int rule, int layout, int width, int temporal_steps, int batch, int* effective_steps,
                             int* private_segment_bytes, int* registers) {
     if (rule < 0 || rule >= RULES) return fail(CELLATO_E_RULE, "unknown rule id " + std::to_string(rule));

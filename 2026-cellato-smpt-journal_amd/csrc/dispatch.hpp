@@ -7,6 +7,9 @@
 #ifndef CELLATO_HIP_DISPATCH_HPP
 #define CELLATO_HIP_DISPATCH_HPP
 
+#include <cstdint>
+
+#include "bit_array.hpp"
 #include "stream.hpp"
 
 namespace cellato::hip {
@@ -36,6 +39,7 @@ struct rule_kernels {
     const void* simple[2][SIMPLE_VARIANTS];                   // [layout][variant]
     const void* temporal[2][TEMPORAL_VARIANTS][MAX_T + 1];    // [layout][variant][T], T >= 1
     const void* batch[MAX_T + 1];                             // linear layout, [T], T >= 1
+    const void* bit_array[2];                                 // bit_array layout, [0] 32-bit, [1] 64-bit words
 };
 
 // MaxT: the deepest temporal kernel instantiated for this layout; deeper entries stay null and
@@ -60,10 +64,11 @@ constexpr void fill_layout(rule_kernels& k) {
     }(std::make_integer_sequence<int, MaxT>{});
 }
 
-// MaxLinearT: the deepest depth whose linear-layout kernels fit the register file without
-// scratch memory (the linear ring slots also hold the neighbours' edge halves, so a rule with
-// many planes runs out of VGPRs at a lower depth than in the tiled layout).
-template <class Config, int MaxLinearT = MAX_T>
+// MaxLinearT / MaxTiledT: the deepest depth whose kernels of that layout fit the register file
+// without scratch memory (every ring slot also holds the neighbours' edge bits: halves of the
+// neighbouring words in the linear layout, an edge word per plane in the tiled one, so a rule
+// with many planes runs out of VGPRs at a lower depth).
+template <class Config, int MaxLinearT = MAX_T, int MaxTiledT = MAX_T>
 rule_kernels make_rule_kernels() {
     using Eval = cellato::evaluators::bit_planes::evaluator<word, typename Config::state_dictionary, typename Config::algorithm>;
     rule_kernels k{};
@@ -74,7 +79,11 @@ rule_kernels make_rule_kernels() {
     for (int i = 0; i < k.nstates && i < 32; ++i)
         k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
     fill_layout<Eval, LINEAR, MaxLinearT>(k);
-    fill_layout<Eval, TILED>(k);
+    fill_layout<Eval, TILED, MaxTiledT>(k);
+    k.bit_array[0] = reinterpret_cast<const void*>(
+        &bit_array_kernel<typename Config::state_dictionary, typename Config::algorithm, std::uint32_t>);
+    k.bit_array[1] = reinterpret_cast<const void*>(
+        &bit_array_kernel<typename Config::state_dictionary, typename Config::algorithm, std::uint64_t>);
     for (int t = 0; t <= MAX_T; ++t) k.batch[t] = nullptr;
     [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
         ((k.batch[Ts + 1] = reinterpret_cast<const void*>(&batch_kernel<Eval, LINEAR, Ts + 1>)), ...);

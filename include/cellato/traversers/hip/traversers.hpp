@@ -8,7 +8,8 @@
 // so a test suite of the reference (include/experiments/test_suites.hpp:116-225) binds them
 // with its namespace alias changed from cellato::traversers::cuda to cellato::traversers::hip.
 // The evaluator is bit_planes::evaluator for the linear grid, tiled_bit_planes::evaluator for
-// the tiled one (checked at compile time: a mismatch would compute a different automaton).
+// the tiled one, bit_array::evaluator<grid, algorithm> for the bit_array grid (checked at
+// compile time: a mismatch would compute a different automaton).
 //
 // Each call goes through the C ABI of include/cellato_hip.h (libcellato_hip.so), which holds
 // the gfx950 kernels of the automata registered in include/cellato/automata/all.hpp; the
@@ -24,9 +25,11 @@
 
 #include "../../../cellato_hip.h"
 #include "../../automata/all.hpp"
+#include "../../evaluators/bit_array.hpp"
 #include "../../evaluators/bit_planes.hpp"
 #include "../../evaluators/tiled_bit_planes.hpp"
 #include "../../experiments/run_params.hpp"
+#include "../../memory/bit_array_grid.hpp"
 #include "../../memory/bit_planes_grid.hpp"
 
 namespace cellato::traversers::hip {
@@ -128,9 +131,83 @@ private:
     bool _ready = false;
 };
 
+// The simple traverser over the bit_array layout (the reference binds it as
+// traversers::simple::traverser<evaluators::bit_array::evaluator<grid_t, algorithm_t>, grid_t>,
+// include/experiments/test_suites.hpp:86-113): one device array of words per buffer, one
+// generation per launch, through cellato_hip_bit_array.
+template <typename evaluator_type, typename grid_type>
+class bit_array_traverser {
+    using evaluator_t = evaluator_type;
+    using grid_t = grid_type;
+    static constexpr int rule = rule_id_of<typename evaluator_t::expression>();
+    static_assert(rule >= 0, "this rule is not compiled into libcellato_hip.so (include/cellato/automata/all.hpp)");
+    static_assert(evaluator_t::layout == grid_t::layout,
+                  "evaluators::bit_array::evaluator goes with grids::bit_array::grid");
+    static constexpr int word_bits = static_cast<int>(sizeof(typename grid_t::store_type) * 8);
+    using device_grid_t = std::invoke_result_t<decltype(&grid_t::to_cuda), grid_t>;
+
+public:
+    static constexpr bool is_CUDA = true;
+    struct no_callback {};
+
+    void init(grid_t grid, const cellato::run::run_params& params) {
+        if (params.word_size != 0 && params.word_size != word_bits)
+            throw std::runtime_error("word_size does not match the grid's word type");
+        _cur = grid.to_cuda();
+        _other = grid.to_cuda();
+        _t = 0;
+        _ready = true;
+    }
+
+    template <typename callback = no_callback>
+    void run(int steps, callback&& cb = no_callback{}) {
+        if (!_ready) throw std::runtime_error("run() before init()");
+        if constexpr (std::is_same_v<std::decay_t<callback>, no_callback>) {
+            advance(steps);
+        } else {
+            auto host = _cur.to_cpu();
+            cb(0, host);
+            for (int k = 0; k < steps; ++k) {
+                advance(1);
+                auto h = _cur.to_cpu();
+                cb(k + 1, h);
+            }
+        }
+    }
+
+    grid_t fetch_result() {
+        cellato::memory::grids::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        grid_t r = _cur.to_cpu();
+        _cur.free_cuda_memory();
+        _other.free_cuda_memory();
+        _ready = false;
+        return r;
+    }
+
+private:
+    void advance(int steps) {
+        int in_b = 0;
+        check(cellato_hip_bit_array(rule, word_bits, _cur.raw_data(), _other.raw_data(),
+                                    static_cast<int>(_cur.x_size_logical()), static_cast<int>(_cur.y_size_logical()), _t,
+                                    steps, nullptr, &in_b));
+        if (in_b) std::swap(_cur, _other);
+        _t += steps;
+    }
+
+    device_grid_t _cur, _other;
+    int _t = 0;
+    bool _ready = false;
+};
+
+template <typename grid_type, typename = void>
+struct is_bit_array_grid : std::false_type {};
+template <typename grid_type>
+struct is_bit_array_grid<grid_type, std::void_t<decltype(grid_type::is_bit_array)>> : std::bool_constant<grid_type::is_bit_array> {};
+
 namespace simple {
 template <typename evaluator_type, typename grid_type>
-using traverser = traverser_base<evaluator_type, grid_type, false>;
+using traverser = std::conditional_t<is_bit_array_grid<grid_type>::value, bit_array_traverser<evaluator_type, grid_type>,
+                                     traverser_base<evaluator_type, grid_type, false>>;
 }
 namespace temporal {
 template <typename evaluator_type, typename grid_type, double average_halo_radius = 1.0>

@@ -75,6 +75,16 @@ int cellato_hip_temporal(int rule, int layout, uint64_t* const* buf_a, uint64_t*
                          int width, int height, int t0, int steps, int temporal_steps,
                          int chunk_rows, void* stream, int* result_in_b);
 
+/* bit_array layout (reference: evaluators::bit_array::evaluator under the CUDA simple traverser,
+ * include/evaluators/bit_array.hpp:73-107, traversers/cuda/simple.cu:20-97): one device array
+ * of word_bits-bit words (32 or 64), cells_per_word = word_bits / P cells of a row per word,
+ * cell i (row-major) at bits [(i % cells_per_word) * P, + P) of word i / cells_per_word, as
+ * its dictionary index; width must be a multiple of cells_per_word. One generation per
+ * launch, every cell evaluated with plain integer semantics; ping-pong between buf_a and
+ * buf_b as for the other traversers (*result_in_b = steps % 2). */
+int cellato_hip_bit_array(int rule, int word_bits, void* buf_a, void* buf_b, int width, int height, int t0, int steps,
+                          void* stream, int* result_in_b);
+
 /* The launch one pass would make (host only, no GPU work): rows per task (`chunk`; chunk_rows
  * 0 = the automatic choice), tasks (one wave each) and blocks of 4 waves. temporal_steps 0 =
  * the simple traverser's kernel, 1..max = the temporal traverser's. No counterpart in the

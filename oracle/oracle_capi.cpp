@@ -142,5 +142,11 @@ void oracle_pack_linear(const int32_t* idx, int W, int H, int P, uint64_t* plane
 void oracle_unpack_linear(const uint64_t* planes, int W, int H, int P, int32_t* idx) { oracle::layouts::unpack_linear(planes, W, H, P, idx); }
 void oracle_pack_tiled(const int32_t* idx, int W, int H, int P, uint64_t* planes) { oracle::layouts::pack_tiled(idx, W, H, P, planes); }
 void oracle_unpack_tiled(const uint64_t* planes, int W, int H, int P, int32_t* idx) { oracle::layouts::unpack_tiled(planes, W, H, P, idx); }
+void oracle_pack_bit_array(const int32_t* idx, int W, int H, int P, int word_bits, uint64_t* words) {
+    oracle::layouts::pack_bit_array(idx, W, H, P, word_bits, words);
+}
+void oracle_unpack_bit_array(const uint64_t* words, int W, int H, int P, int word_bits, int32_t* idx) {
+    oracle::layouts::unpack_bit_array(words, W, H, P, word_bits, idx);
+}
 
 } // extern "C"

@@ -1,8 +1,8 @@
 // GPU test of the C++ host API (include/cellato/traversers/hip/traversers.hpp).
 //
-// The four GPU test-suite shapes of the reference (include/experiments/test_suites.hpp:116-225:
-// bit_planes, tiled_bit_planes, temporal_tiled_bit_planes, temporal_linear_bit_planes) are bound
-// below with the same type expressions, the traversers' namespace alias pointing at
+// The five GPU bit-level test-suite shapes of the reference (include/experiments/test_suites.hpp:
+// 86-225: bit_array, bit_planes, tiled_bit_planes, temporal_tiled_bit_planes,
+// temporal_linear_bit_planes) are bound below with the same type expressions, the traversers' namespace alias pointing at
 // cellato::traversers::hip instead of cellato::traversers::cuda. Each is driven the way the
 // reference's experiment manager drives a suite (experiment_manager.hpp:61-97): standard grid
 // -> grid_t by implicit conversion -> init -> run -> fetch_result -> to_standard -> checksum,
@@ -13,8 +13,10 @@
 #include <vector>
 
 #include "cellato/automata/all.hpp"
+#include "cellato/evaluators/bit_array.hpp"
 #include "cellato/evaluators/bit_planes.hpp"
 #include "cellato/evaluators/tiled_bit_planes.hpp"
+#include "cellato/memory/bit_array_grid.hpp"
 #include "cellato/memory/bit_planes_grid.hpp"
 #include "cellato/traversers/hip/traversers.hpp"
 #include "standard.hpp"   // oracle (the checker)
@@ -26,6 +28,18 @@ namespace traversers = cellato::traversers::hip;   // the reference: cellato::tr
 
 template <typename store_word_type>
 struct using_ {
+    template <typename cellular_automaton>
+    struct bit_array {
+        using original_cell_t = typename cellular_automaton::cell_state;
+        using grid_store_word_t = store_word_type;
+        using algorithm_t = typename cellular_automaton::algorithm;
+        using state_dictionary_t = typename cellular_automaton::state_dictionary;
+        using grid_t = grids::bit_array::grid<state_dictionary_t, grid_store_word_t>;
+        using evaluator_t = evaluators::bit_array::evaluator<grid_t, algorithm_t>;
+        using traverser_t = traversers::simple::traverser<evaluator_t, grid_t>;
+        static constexpr const char* what = sizeof(store_word_type) == 8 ? "bit_array simple 64" : "bit_array simple 32";
+    };
+
     template <typename cellular_automaton>
     struct bit_planes {
         using original_cell_t = typename cellular_automaton::cell_state;
@@ -92,6 +106,7 @@ bool check(int W, int H, int steps, int T) {
     typename S::traverser_t t;
     run::run_params p;
     p.temporal_steps = T;
+    p.word_size = static_cast<int>(sizeof(typename grid_t::store_type) * 8);   // as --word_size sets it
     t.init(grid, p);
     t.run(steps / 2);
     t.run(steps - steps / 2);
@@ -108,6 +123,8 @@ bool check(int W, int H, int steps, int T) {
     return got == want;
 }
 
+template <class C> using ba64 = suites::using_<word>::bit_array<C>;
+template <class C> using ba32 = suites::using_<std::uint32_t>::bit_array<C>;
 template <class C> using lin = suites::using_<word>::bit_planes<C>;
 template <class C> using til = suites::using_<word>::tiled_bit_planes<C>;
 template <class C> using ttil = suites::using_<word>::temporal_tiled_bit_planes<C>;
@@ -115,6 +132,11 @@ template <class C> using tlin = suites::using_<word>::temporal_linear_bit_planes
 
 int main() {
     bool ok = true;
+    ok &= check<ba64, game_of_life::config>(1024, 48, 13, 0);
+    ok &= check<ba32, wire::config>(512, 40, 11, 0);
+    ok &= check<ba64, critters::config>(640, 36, 13, 0);
+    ok &= check<ba32, cyclic::config>(600, 30, 9, 0);      // 6 cells per 32-bit word
+    ok &= check<ba64, excitable::config>(630, 33, 9, 0);   // 21 cells per 64-bit word
     ok &= check<lin, game_of_life::config>(8192, 96, 13, 0);
     ok &= check<lin, excitable::config>(4160, 40, 9, 0);
     ok &= check<til, wire::config>(520, 40, 13, 0);

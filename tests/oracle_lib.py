@@ -33,6 +33,8 @@ def lib():
             getattr(L, f).argtypes = [_I32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64]
         for f in ("oracle_unpack_linear", "oracle_unpack_tiled"):
             getattr(L, f).argtypes = [_U64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I32]
+        L.oracle_pack_bit_array.argtypes = [_I32] + [ctypes.c_int] * 4 + [_U64]
+        L.oracle_unpack_bit_array.argtypes = [_U64] + [ctypes.c_int] * 4 + [_I32]
         _lib = L
     return _lib
 
@@ -92,3 +94,19 @@ def checksum(values):
     out = np.zeros(16, np.uint64)
     lib().oracle_checksum(v.reshape(-1), W, H, out)
     return [int(x) for x in out]
+
+
+def pack_bit_array(indices, P, word_bits):
+    """The bit_array words of a grid of dictionary indices (one word per uint64 slot)."""
+    idx = np.ascontiguousarray(indices, np.int32)
+    H, W = idx.shape
+    cpw = word_bits // P
+    out = np.zeros((W * H + cpw - 1) // cpw, np.uint64)
+    lib().oracle_pack_bit_array(idx.reshape(-1), W, H, P, word_bits, out)
+    return out
+
+
+def unpack_bit_array(words, W, H, P, word_bits):
+    out = np.zeros(W * H, np.int32)
+    lib().oracle_unpack_bit_array(np.ascontiguousarray(words, np.uint64), W, H, P, word_bits, out)
+    return out.reshape(H, W)

@@ -114,3 +114,17 @@ def test_multi_static_refuses_tables_beyond_its_scratch():
     rc = L.cellato_hip_multi_static(arr, 1, 0, None, in_b)
     assert rc == _lib.E_ARG
     assert b"cellato_hip_multi_static's scratch" in L.cellato_hip_last_error()
+
+
+def test_bit_array_rejects_widths_that_do_not_fill_words():
+    """cellato_hip_bit_array checks its geometry on the host: a row must be whole words, words
+    are 32 or 64 bits."""
+    with pytest.raises(_lib.CellatoError) as e:
+        _lib.bit_array(3, 64, FAKE[0], FAKE[1], 16384, 8, 0, 1)   # 3 planes: 21 cells per 64-bit word
+    assert e.value.code == _lib.E_GEOMETRY and "cells_per_word" in str(e.value)
+    with pytest.raises(_lib.CellatoError) as e:
+        _lib.bit_array(0, 16, FAKE[0], FAKE[1], 64, 8, 0, 1)
+    assert e.value.code == _lib.E_ARG
+    with pytest.raises(_lib.CellatoError) as e:
+        _lib.bit_array(11, 64, FAKE[0], FAKE[1], 64, 8, 0, 1)
+    assert e.value.code == _lib.E_RULE

@@ -76,15 +76,18 @@ def parse(line):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("rule,evaluator,traverser,T", [
-    ("game-of-life", "bit_planes", "temporal", 4),
-    ("wire", "tiled_bit_planes", "simple", 0),
-    ("critters", "bit_planes", "temporal", 3),
+@pytest.mark.parametrize("rule,evaluator,traverser,T,word,size", [
+    ("game-of-life", "bit_planes", "temporal", 4, 64, (16384, 16384)),
+    ("wire", "tiled_bit_planes", "simple", 0, 64, (16384, 16384)),
+    ("critters", "bit_planes", "temporal", 3, 64, (16384, 16384)),
+    ("brian", "bit_array", "simple", 0, 32, (16384, 16384)),
+    ("excitable", "bit_array", "simple", 0, 64, (16128, 16644)),   # 21 cells per 64-bit word
 ])
-def test_published_checksum_from_the_command_line(rule, evaluator, traverser, T):
-    g = next(e for e in GOLD if e["automaton"] == rule and e["x_size"] == e["y_size"] == 16384 and e["steps"] == 128)
+def test_published_checksum_from_the_command_line(rule, evaluator, traverser, T, word, size):
+    W, H = size
+    g = next(e for e in GOLD if e["automaton"] == rule and (e["x_size"], e["y_size"], e["steps"]) == (W, H, 128))
     args = ["--automaton", rule, "--device", "CUDA", "--traverser", traverser, "--evaluator", evaluator,
-            "--layout", evaluator, "--x_size", 16384, "--y_size", 16384, "--steps", 128, "--word_size", 64,
+            "--layout", evaluator, "--x_size", W, "--y_size", H, "--steps", 128, "--word_size", word,
             "--seed", g["seed"], "--rounds", 2, "--warmup_rounds", 1]
     if traverser == "temporal":
         args += ["--temporal_steps", T, "--temporal_tile_size_y", 32]

@@ -31,3 +31,16 @@ def test_callback_mode_sees_every_generation():
     assert [i for i, _ in seen] == [0, 1, 2, 3, 4]
     for i, g in seen:
         np.testing.assert_array_equal(g, O.run_standard("traffic", v, i))
+
+
+def test_bit_array_traverser_lifecycle():
+    import cellato_amd as C
+    for rule, wb, w, h in [("cyclic", 32, 600, 20), ("critters", 64, 640, 24), ("forest-fire", 64, 512, 9)]:
+        v = C.iid_values(rule, w, h, 4)
+        tr = C.BitArrayTraverser()
+        tr.init(C.BitArrayGrid.from_standard(rule, v, wb), C.RunParams(automaton=rule, word_size=wb))
+        tr.run(4)
+        tr.run(7)   # continues at time step 4
+        out = tr.fetch_result()
+        np.testing.assert_array_equal(out.to_standard(), O.run_standard(rule, v, 11))
+        assert out.x_size_physical() == w // (wb // C.rule(rule).planes)

@@ -1,5 +1,5 @@
-"""The C++ host API's types, checked with the host compiler (no GPU): the reference's four GPU
-test-suite type expressions (include/experiments/test_suites.hpp:116-225) compile against
+"""The C++ host API's types, checked with the host compiler (no GPU): the reference's five GPU
+bit-level test-suite type expressions (include/experiments/test_suites.hpp:86-225) compile against
 include/cellato with only the traversers' namespace changed, and misuse fails at compile time
 with a message (an evaluator of one layout on a grid of the other, a halo radius the kernels do
 not carry)."""
@@ -13,6 +13,7 @@ REPO = Path(__file__).resolve().parent.parent
 
 PRELUDE = """
 #include "cellato/automata/all.hpp"
+#include "cellato/evaluators/bit_array.hpp"
 #include "cellato/evaluators/bit_planes.hpp"
 #include "cellato/evaluators/tiled_bit_planes.hpp"
 #include "cellato/memory/bit_planes_grid.hpp"
@@ -28,6 +29,8 @@ using lin_grid = grids::bit_planes::grid<word, state_dictionary_t>;
 using til_grid = grids::tiled_bit_planes::grid<word, state_dictionary_t>;
 using lin_eval = evaluators::bit_planes::evaluator<word, state_dictionary_t, algorithm_t>;
 using til_eval = evaluators::tiled_bit_planes::evaluator<word, state_dictionary_t, algorithm_t>;
+using ba_grid = grids::bit_array::grid<state_dictionary_t, unsigned int>;
+using ba_eval = evaluators::bit_array::evaluator<ba_grid, algorithm_t>;
 """
 
 
@@ -50,6 +53,9 @@ void f() {
     use<traversers::simple::traverser<til_eval, til_grid>>();
     use<traversers::temporal::traverser<til_eval, til_grid, ca::average_halo_radius>>();
     use<traversers::temporal::traverser<lin_eval, lin_grid, ca::average_halo_radius>>();
+    use<traversers::simple::traverser<ba_eval, ba_grid>>();
+    static_assert(ba_grid::cells_per_word == 16 && ba_grid::bits_per_cell == 2 && ba_grid::cell_mask == 3u);
+    static_assert(grids::bit_array::grid<state_dictionary_t>::cells_per_word == 16);   // 32-bit words by default
     // the grid members the suites and the reference traversers read
     static_assert(lin_grid::x_word_tile_size == 64 && lin_grid::y_word_tile_size == 1);
     static_assert(til_grid::x_word_tile_size == 8 && til_grid::y_word_tile_size == 8);
@@ -75,3 +81,8 @@ def test_layout_mismatch_is_a_compile_error(tmp_path):
 def test_halo_radius_beyond_one_cell_is_a_compile_error(tmp_path):
     rc, err = compile_snippet("traversers::temporal::traverser<lin_eval, lin_grid, 2.0> t;\n", tmp_path)
     assert rc != 0 and "halo of one cell per generation" in err
+
+
+def test_bit_array_grid_mismatch_is_a_compile_error(tmp_path):
+    rc, err = compile_snippet("traversers::simple::traverser<lin_eval, ba_grid> t;\n", tmp_path)
+    assert rc != 0
