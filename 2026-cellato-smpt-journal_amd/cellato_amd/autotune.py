@@ -1,12 +1,12 @@
-"""Autotuner: the temporal depth T per (automaton, size).
+"""Autotuner: the launch shape per (automaton, size): temporal depth T and rows per task
+(the tile shape: a wave's chunk of rows, 0 = the planner's choice, csrc/plan.hpp auto_chunk).
 
-The rows per task are not tuned here: the launch planner picks them per launch round from
-the round's total work (csrc/plan.hpp, auto_chunk). A depth is measured as throughput, the
-way a batch uses it: enough copies of a grid of the bucket's size to fill the GPU, advanced
-by one graph launch (`Batch`), timed with HIP events over a few replays. Sizes are bucketed
-by powers of two of width and height. Without a cost model every depth of a bucket is
-measured; with one (`cellato_amd.costmodel.CostModel`) only the `budget` depths it ranks
-best are measured. Results are cached in a JSON file.
+A candidate (T, chunk) is measured as throughput, the way a batch uses it: enough copies of a
+grid of the bucket's size to fill the GPU, advanced by one graph launch (`Batch`), timed with
+HIP events over a few replays. Sizes are bucketed by powers of two of width and height.
+Without a cost model every candidate of a bucket is measured; with one
+(`cellato_amd.costmodel.CostModel`) only the `budget` candidates it ranks best are measured.
+Results are cached in a JSON file.
 """
 import json
 import math
@@ -17,25 +17,47 @@ import numpy as np
 from .rules import rule as get_rule
 
 DEPTHS = (1, 2, 3, 4, 5, 6, 8)
+CHUNKS = (0, 32, 128)          # rows per task: the planner's choice, short and tall tiles
 BATCH_DEPTHS = (2, 4, 6, 8)
-CANDIDATES = [(T, 0) for T in DEPTHS]
+CANDIDATES = [(T, c) for T in DEPTHS for c in CHUNKS]
 
 
 def bucket(side):
     return int(2 ** max(6, min(15, round(math.log2(max(side, 1))))))
 
 
+_KERNELS = {}
+
+
+def kernel_shape(rule_name, width, T):
+    """(depth the pass runs at, registers per lane) of the kernel a pass of up to T generations
+    over rows `width` cells wide launches (cellato_hip_kernel_info; needs the GPU)."""
+    from . import _lib
+    wx = width // 64
+    key = (rule_name, 0 if wx < 64 else 1 if wx == 64 else 2, T)
+    if key not in _KERNELS:
+        eff, _, regs = _lib.kernel_info(get_rule(rule_name).id, 0, width, T)
+        _KERNELS[key] = (eff, regs)
+    return _KERNELS[key]
+
+
 def features(rule_name, width, height, T, chunk=0):
     """Inputs of the cost model: what the rule is made of (planes and the kinds of nodes of
-    its AST, not its identity, so that the model says something about rules it has not
-    seen), the grid size, the depth, and the lane mapping the size implies."""
+    its AST, not its identity), what the compiler made of it at this depth (the kernel's
+    registers per lane, the waves per SIMD they allow, the depth the pass really runs at),
+    so that the model says something about rules it has not seen; the grid size, the depth,
+    the rows per task, and the lane mapping the size implies."""
     r = get_rule(rule_name)
     wx = width // 64
+    eff, regs = kernel_shape(rule_name, width, T)
+    waves = min(8, 512 // max(regs, 1))
     return [r.planes / 5.0, math.log2(width) / 15.0, math.log2(height) / 15.0, T / 8.0, 1.0 / T,
             (T / 8.0) ** 2, r.planes * T / 40.0,
             float(r.has("moore")), float(r.has("von_neumann")), float(r.has("margolus")),
             float(r.has("neighbor")), float(r.has("dynamic_target")), float(r.has("alternating")),
-            1.0 if wx < 64 else 0.0, 1.0 if wx == 64 else 0.0]
+            1.0 if wx < 64 else 0.0, 1.0 if wx == 64 else 0.0,
+            regs / 512.0, waves / 8.0, eff / 8.0,
+            0.0 if chunk == 0 else math.log2(chunk) / 10.0, 1.0 if chunk == 0 else 0.0]
 
 
 class Tuner:
@@ -103,9 +125,9 @@ class Tuner:
         measured T, else the cost model's prediction (tuning the bucket first if needed)."""
         self.pick(rule_name, width, height)
         e = self.cache.get(self.key(rule_name, width, height))
-        for m in (e or {}).get("measured", []):
-            if m["T"] == T and m["gcups"] > 0:
-                return 1.0 / m["gcups"]
+        best = [m["gcups"] for m in (e or {}).get("measured", []) if m["T"] == T and m["gcups"] > 0]
+        if best:
+            return 1.0 / max(best)
         if self.model is not None:
             x = np.array([features(rule_name, bucket(width), bucket(height), T)], np.float32)
             return float(np.exp(self.model.predict(x)[0]))
@@ -132,6 +154,10 @@ class Tuner:
         cands["per_automaton_and_size"] = [self.pick(j.rule, j.width, j.height)[0] for j in jobs]
         return cands
 
+    def batch_chunks(self, jobs):
+        """Rows per task of each job at its bucket's tuned tile shape (0 = the planner's)."""
+        return [self.pick(j.rule, j.width, j.height)[1] for j in jobs]
+
     def batch_depths(self, jobs):
         """Depth per job of a batch, measured on the batch itself: every candidate of
         `batch_candidates` is built (`Batch`) and replayed once after a warm-up replay, and
@@ -141,14 +167,15 @@ class Tuner:
         winner, {candidate: GCUPS})."""
         import torch
         from .batch import Batch
-        saved = [j.temporal_steps for j in jobs]
+        saved = [(j.temporal_steps, j.chunk_rows) for j in jobs]
         cu = sum(j.cells * j.steps for j in jobs)
         cands = self.batch_candidates(jobs)
+        chunks = {"per_automaton_and_size": self.batch_chunks(jobs)}   # the buckets' tuned tile shapes
         out = {}
         try:
             for name, depths in cands.items():
-                for j, T in zip(jobs, depths):
-                    j.temporal_steps = T
+                for j, T, c in zip(jobs, depths, chunks.get(name, [0] * len(jobs))):
+                    j.temporal_steps, j.chunk_rows = T, c
                 b = Batch(jobs)
                 b.run()
                 torch.cuda.synchronize()
@@ -162,9 +189,12 @@ class Tuner:
                 del b
                 self.measurements += 1
         finally:
-            for j, t in zip(jobs, saved):
-                j.temporal_steps = t
+            for j, (t, c) in zip(jobs, saved):
+                j.temporal_steps, j.chunk_rows = t, c
         best = max(out, key=out.get)
+        if best in chunks:
+            for j, c in zip(jobs, chunks[best]):
+                j.chunk_rows = c
         return cands[best], best, out
 
     def tune(self, rule_name, width, height):

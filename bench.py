@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--automata", default="game-of-life,wire")
     ap.add_argument("--depths", default="1,2,4,8", help="temporal depths to measure")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--layout", choices=["linear", "tiled"], default="linear",
+                    help="bit-plane layout of the variants phase (the headline is linear)")
+    ap.add_argument("--rounds", type=int, default=5, help="timed rounds per variant, each from the initial state")
+    ap.add_argument("--roofline-launches", type=int, default=200,
+                    help="back-to-back launches timed for the headline kernel's average launch duration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the 2048^2 bit-exactness check")
     ap.add_argument("--no-simple", action="store_true", help="measure the temporal depths only")
@@ -63,7 +68,10 @@ def parse():
     ap.add_argument("--no-multiplane", action="store_true", help="skip the multi-plane rules")
     ap.add_argument("--multiplane", default="cyclic,excitable,fluid,critters")
     ap.add_argument("--multiplane-steps", type=int, default=400)
-    ap.add_argument("--phase", choices=["variants", "multiplane", "batch", "serve", "cpu", "table"], default=None,
+    ap.add_argument("--no-bit-array", action="store_true", help="skip the bit_array layout")
+    ap.add_argument("--bit-array-steps", type=int, default=128)
+    ap.add_argument("--phase", choices=["variants", "multiplane", "bit_array", "batch", "serve", "cpu", "table"],
+                    default=None,
                     help="run one phase in this process (bench.py runs each phase in a child process)")
     ap.add_argument("--tuner-cache", default=None, help="autotuner measurements shared by the phases")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -72,39 +80,93 @@ def parse():
     return ap.parse_args()
 
 
-def time_variant(C, torch, rule, dev, kind, T, steps, warmup, chunk):
-    """GCUPS and per-launch average (ms) of one traverser variant; the grid stays on the device."""
-    stream = torch.cuda.current_stream().cuda_stream
+def time_variant(C, torch, rule, dev, init, kind, T, steps, warmup, rounds, chunk):
+    """One traverser variant timed as the reference times a suite (experiment_manager.hpp:61-148):
+    every round starts from the same initial state (`init`, copied in outside the timed region),
+    `warmup` generations run untimed first, then `rounds` timed rounds of `steps` generations,
+    each bracketed by HIP events on the launch stream, and each round's result is checksummed on
+    the device (16 tiles); the rounds must agree. The grid stays on the device throughout."""
+    stream = torch.cuda.current_stream()
     W, H = dev.width, dev.height
+
+    def reset():
+        dev.buf[0].copy_(init)
+        dev.cur = 0
 
     def advance(n, t0):
         if n <= 0:
             return
         if kind == "simple":
-            in_b = C._lib.simple(rule.id, 0, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, t0, n, chunk, stream)
+            in_b = C._lib.simple(rule.id, dev.layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, t0, n, chunk,
+                                 stream.cuda_stream)
         else:
-            in_b = C._lib.temporal(rule.id, 0, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, t0, n, T, chunk, stream)
+            in_b = C._lib.temporal(rule.id, dev.layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, t0, n, T,
+                                   chunk, stream.cuda_stream)
         if in_b:
             dev.cur = 1 - dev.cur
 
+    reset()
     advance(warmup, 0)
     torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    advance(steps, warmup)
-    stop.record()
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(stop)
+    times, sums = [], []
+    for _ in range(rounds):
+        reset()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        advance(steps, 0)
+        stop.record(stream)
+        stop.synchronize()
+        times.append(start.elapsed_time(stop))
+        sums.append(dev.checksum_device())
+    import numpy as np
+    ms = float(np.mean(times))
     launches = steps if kind == "simple" else -(-steps // T)
-    return W * H * steps / (ms * 1e-3) / 1e9, ms, ms / launches, launches
+    return {"gcups": W * H * steps / (ms * 1e-3) / 1e9, "ms_mean": ms, "ms_std": float(np.std(times)),
+            "ms_rounds": [round(t, 4) for t in times], "ms_per_launch": ms / launches, "launches": launches,
+            "checksum_consistent": all(x == sums[0] for x in sums), "checksum": sums[0]}
 
 
-def check_subset(C, O, rule_name, kind, T):
+def time_launches(C, torch, rule, dev, kind, T, launches, chunk):
+    """Average duration of one launch of a variant's kernel: `launches` launches back to back on
+    one stream, bracketed by HIP events (the launch-to-launch time the kernel trace reports as its
+    average duration; short windows carry the slower first launches)."""
+    stream = torch.cuda.current_stream()
+    W, H = dev.width, dev.height
+    n = launches * (1 if kind == "simple" else T)
+
+    def advance(k):
+        if kind == "simple":
+            in_b = C._lib.simple(rule.id, dev.layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, 0, k, chunk,
+                                 stream.cuda_stream)
+        else:
+            in_b = C._lib.temporal(rule.id, dev.layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, 0, k, T,
+                                   chunk, stream.cuda_stream)
+        if in_b:
+            dev.cur = 1 - dev.cur
+
+    advance(4 * (1 if kind == "simple" else T))
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    advance(n)
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / launches
+
+
+_SUBSET_WANT = {}
+
+
+def check_subset(C, O, rule_name, kind, T, layout=0):
+    """The variant on a 2048^2 subset (seeded i.i.d., 64 generations) against the CPU standard
+    evaluator (one oracle run per automaton, shared by its variants)."""
+    import numpy as np
     from gpu_util import run_gpu
     v = C.iid_values(rule_name, 2048, 2048, 11)
-    want = O.run_bitplanes(rule_name, v, 64)
-    import numpy as np
-    return bool(np.array_equal(run_gpu(rule_name, v, 0, kind, T, 64), want))
+    if rule_name not in _SUBSET_WANT:
+        _SUBSET_WANT[rule_name] = O.run_standard(rule_name, v, 64, threads=min(16, os.cpu_count() or 1))
+    return bool(np.array_equal(run_gpu(rule_name, v, layout, kind, T, 64), _SUBSET_WANT[rule_name]))
 
 
 def cpu_baseline(C, O):
@@ -143,7 +205,7 @@ def make_tuner(cache_path=None):
     from cellato_amd.costmodel import CostModel
     from cellato_amd.tuning_eval import load
     rows, _ = load()
-    X = np.array([features(r["rule"], r["width"], r["height"], r["T"]) for r in rows], np.float32)
+    X = np.array([features(r["rule"], r["width"], r["height"], r["T"], r.get("chunk", 0)) for r in rows], np.float32)
     y = np.log(np.array([r["ns_per_cell"] for r in rows], np.float32))
     t = time.perf_counter()
     model = CostModel(X.shape[1], hidden=(64, 64), seed=0)
@@ -266,21 +328,24 @@ def bench_serve(C, O, torch, tuner, n, rate, limit, tick_passes):
             "p99_vs_one_at_a_time": round(float(np.percentile(lat_all, 99)) / s["p99_s"], 2) if s["p99_s"] else None}
 
 
-def valu_roofline(g, size):
+def valu_roofline(g, size, ms_per_launch):
     """Temporal launches are bound by VALU issue, not HBM: wave64 VALU instructions per launch
     (rocprofv3 SQ_INSTS_VALU, a fixed count for the configuration) over the live average
     launch duration, against the issue peak."""
     ins = measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], size, "valu_wave_instructions")
     if not ins:
         return None
-    rate = ins / (g["ms_per_launch"] * 1e-3)
+    rate = ins / (ms_per_launch * 1e-3)
     return {"achieved": round(rate / 1e9, 1), "peak": round(VALU_PEAK_INSTR_PER_S / 1e9, 1),
             "unit": "G wave64 VALU instructions/s", "frac": round(rate / VALU_PEAK_INSTR_PER_S, 4),
             "instructions_per_launch": ins}
 
 
 def phase_variants(a):
-    """The plain-GPU configuration: every (automaton, traverser, depth) variant at a.size^2."""
+    """The plain-GPU configuration: every (automaton, traverser, depth) variant at a.size^2, timed
+    as warm-up plus a.rounds rounds from the same initial state; then the headline variant's
+    kernel alone over a.roofline_launches back-to-back launches for its average launch duration."""
+    import numpy as np
     import torch
     import cellato_amd as C
     import oracle_lib as O
@@ -289,51 +354,84 @@ def phase_variants(a):
     variants = []
     best = None
     depths = [int(d) for d in a.depths.split(",") if d]
+    grids = {}
+    layout = 1 if a.layout == "tiled" else 0
     for name in a.automata.split(","):
         rule = C.rule(name)
-        v = C.iid_values(name, a.size, a.size, 2026)
-        planes = C.grid.pack(rule.index_of(v), rule.planes, 0)
-        del v
-        dev = C.DeviceGrid(rule, a.size, a.size, 0)
+        dev = C.DeviceGrid(rule, a.size, a.size, layout)
+        if a.size <= 16384:
+            v = C.iid_values(name, a.size, a.size, 2026)
+            dev.upload(C.grid.pack(rule.index_of(v), rule.planes, layout))
+            del v
+        else:   # larger grids: the same distribution drawn on the device
+            dev.fill_random(2026)
+        init = dev.buf[0].clone()
         for kind, T in ([] if a.no_simple else [("simple", 1)]) + [("temporal", d) for d in depths]:
-            dev.upload(planes)
-            dev.cur = 0
-            gcups, ms, ms_launch, launches = time_variant(C, torch, rule, dev, kind, T, a.steps, a.warmup, a.chunk)
+            t = time_variant(C, torch, rule, dev, init, kind, T, a.steps, a.warmup, a.rounds, a.chunk)
             algo_bytes = 2 * rule.planes * a.size * a.size / 8   # P bits read + P bits written per cell and launch
-            rec = {"automaton": name, "traverser": kind, "temporal_steps": T, "gcups": round(gcups, 1),
-                   "ms_total": round(ms, 3), "ms_per_launch": round(ms_launch, 5), "launches": launches,
-                   "algorithmic_GBps": round(algo_bytes / (ms_launch * 1e-3) / 1e9, 1)}
+            rec = {"automaton": name, "traverser": kind, "temporal_steps": T, "gcups": round(t["gcups"], 1),
+                   "rounds": a.rounds, "ms_mean": round(t["ms_mean"], 4), "ms_std": round(t["ms_std"], 4),
+                   "ms_rounds": t["ms_rounds"], "ms_per_launch": round(t["ms_per_launch"], 5),
+                   "launches_per_round": t["launches"], "checksum_consistent": t["checksum_consistent"],
+                   "algorithmic_GBps": round(algo_bytes / (t["ms_per_launch"] * 1e-3) / 1e9, 1)}
+            # average launch duration over back-to-back launches (what a kernel trace reports)
+            ms_b2b = time_launches(C, torch, rule, dev, kind, T, a.roofline_launches, a.chunk)
+            rec["us_per_launch_b2b"] = round(ms_b2b * 1e3, 3)
+            rec["algorithmic_GBps_b2b"] = round(algo_bytes / (ms_b2b * 1e-3) / 1e9, 1)
+            if layout:
+                rec["layout"] = "tiled"
             if not a.no_check:
-                rec["bit_exact_2048_subset"] = check_subset(C, O, name, kind, T)
+                rec["bit_exact_2048_subset"] = check_subset(C, O, name, kind, T, layout)
             print(json.dumps(rec), file=sys.stderr, flush=True)
             variants.append(rec)
-            if name == "game-of-life" and (best is None or gcups > best["gcups"]):
+            if name == "game-of-life" and (best is None or rec["gcups"] > best["gcups"]):
                 best = rec
-        del dev
+        grids[name] = (rule, dev)
+        del init
         torch.cuda.empty_cache()
+
+    g = best or variants[0]
+    rule, dev = grids[g["automaton"]]
+    bytes_per_launch = 2 * rule.planes * a.size * a.size // 8
+    ms_launch, achieved = g["us_per_launch_b2b"] * 1e-3, g["algorithmic_GBps_b2b"]
+    valu = valu_roofline(g, a.size, ms_launch)
+    simple_gol = next((r for r in variants if r["automaton"] == g["automaton"] and r["traverser"] == "simple"), None)
+    simple_roof = None
+    if simple_gol is not None:
+        gbps_s = simple_gol["algorithmic_GBps_b2b"]
+        simple_roof = {"bound": "hbm", "achieved": gbps_s, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                       "frac": round(gbps_s / HBM_PEAK_GBPS, 4), "kernel": "stream_kernel (simple)",
+                       "us_per_launch": simple_gol["us_per_launch_b2b"], "launches_timed": a.roofline_launches,
+                       "bytes_per_launch": bytes_per_launch,
+                       "traffic": measured_launch(g["automaton"], "simple", 1, a.size)}
+    for name, (r, d) in grids.items():
+        del d
+    grids.clear()
+    torch.cuda.empty_cache()
 
     copy_rec = None
     if a.copy_mib:
+        # calibration kernel of the FETCH_SIZE / WRITE_SIZE counters: a plain elementwise pass
+        # that reads a.copy_mib MiB and writes a.copy_mib MiB (torch's bitwise_not, a kernel
+        # nothing else in the run launches, so that its counters are not mixed with others)
         n = a.copy_mib * 2**20 // 8
-        src = torch.empty(n, dtype=torch.int64, device="cuda")
+        src = torch.zeros(n, dtype=torch.int64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.bitwise_not(src, out=dst)
         torch.cuda.synchronize()
         t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0.record()
         for _ in range(20):
-            dst.copy_(src)
+            torch.bitwise_not(src, out=dst)
         t1.record()
         torch.cuda.synchronize()
         ms = t0.elapsed_time(t1) / 20
-        copy_rec = {"copy_MiB": a.copy_mib, "GBps": round(2 * a.copy_mib * 2**20 / ms / 1e6, 1)}
+        copy_rec = {"copy_MiB": a.copy_mib, "kernel": "torch bitwise_not (read + write)",
+                    "GBps": round(2 * a.copy_mib * 2**20 / ms / 1e6, 1)}
         print(json.dumps(copy_rec), file=sys.stderr, flush=True)
         del src, dst
 
-    g = best or variants[0]
-    achieved = g["algorithmic_GBps"]
-    simple_gol = next((r for r in variants if r["automaton"] == g["automaton"] and r["traverser"] == "simple"), None)
     out = {
         "metric": "GCUPS (cell updates/s) at 16384^2 bit-exact; batched/served vs plain GPU; p99",
         "value": g["gcups"],
@@ -341,25 +439,30 @@ def phase_variants(a):
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(g["ms_total"] / a.steps, 5),
+        "rounds": a.rounds,
+        "ms_per_step": round(g["ms_mean"] / a.steps, 5),
+        "ms_per_step_std": round(g["ms_std"] / a.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {"workload": f"{g['automaton']} {a.size}x{a.size}, i.i.d. seeded, {a.steps} generations, "
-                               f"plain GPU path (one grid), linear bit planes, 64-bit words",
+        "config": {"workload": f"{g['automaton']} {a.size}x{a.size}, i.i.d. seeded, {a.steps} generations per round, "
+                               f"{a.rounds} rounds from the same initial state after {a.warmup} warm-up generations, "
+                               f"plain GPU path (one grid), {a.layout} bit planes, 64-bit words",
                    "traverser": g["traverser"], "temporal_steps": g["temporal_steps"]},
         "bit_exact": all(r.get("bit_exact_2048_subset", True) for r in variants),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "checksum_consistent": all(r["checksum_consistent"] for r in variants),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], a.size),
                      "kernel": f"stream_kernel ({g['traverser']}, T={g['temporal_steps']})",
-                     "bytes_per_launch": 2 * C.rule(g["automaton"]).planes * a.size * a.size // 8,
-                     "valu": valu_roofline(g, a.size)},
-        "roofline_simple": None if simple_gol is None else {
-            "bound": "hbm", "achieved": simple_gol["algorithmic_GBps"], "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s", "frac": round(simple_gol["algorithmic_GBps"] / HBM_PEAK_GBPS, 4)},
+                     "bytes_per_launch": bytes_per_launch,
+                     "us_per_launch": round(ms_launch * 1e3, 3), "launches_timed": a.roofline_launches,
+                     # at T > 1 generations per launch the kernel is limited by VALU issue, not HBM
+                     "limiter": "valu" if g["temporal_steps"] > 1 and g["traverser"] == "temporal" else "hbm",
+                     "valu": valu},
+        "roofline_simple": simple_roof,
         "variants": variants,
     }
     if copy_rec:
@@ -376,32 +479,107 @@ H100_BEST = {"cyclic": 1505, "excitable": 3146, "fluid": 9197, "critters": 4680}
 def phase_multiplane(a):
     """Multi-plane rules on the plain GPU path: Cyclic (5 planes), Greenberg-Hastings (3),
     HPP fluid (4), Critters (1, Margolus), 16384^2, seeded i.i.d., temporal traverser at
-    depths 2/4/8; per automaton the best depth, each re-run bit-exact on a 2048^2 subset."""
+    depths 2/4/8, each timed as warm-up plus a.rounds rounds from the same initial state; per
+    automaton the best depth, re-run bit-exact on a 2048^2 subset against the standard
+    evaluator. A depth whose kernel would spill registers runs at the deepest one that does
+    not (`temporal_steps_used`)."""
     import torch
     import cellato_amd as C
     import oracle_lib as O
+    from cellato_amd import _lib
     res = {}
     for name in a.multiplane.split(","):
         rule = C.rule(name)
         dev = C.DeviceGrid(rule, a.size, a.size, 0)
-        planes = C.grid.pack(rule.index_of(C.iid_values(name, a.size, a.size, 2026)), rule.planes, 0)
-        runs = {}
+        dev.upload(C.grid.pack(rule.index_of(C.iid_values(name, a.size, a.size, 2026)), rule.planes, 0))
+        init = dev.buf[0].clone()
+        runs, used, std, consistent = {}, {}, {}, True
         for T in (2, 4, 8):
-            dev.upload(planes)
-            dev.cur = 0
-            gcups, ms, ms_launch, _ = time_variant(C, torch, rule, dev, "temporal", T, a.multiplane_steps, 10, 0)
-            runs[T] = round(gcups, 1)
+            t = time_variant(C, torch, rule, dev, init, "temporal", T, a.multiplane_steps, 10, a.rounds, 0)
+            runs[T] = round(t["gcups"], 1)
+            std[T] = round(t["ms_std"] / t["ms_mean"], 4)
+            used[T] = _lib.kernel_info(rule.id, 0, a.size, T)[0]
+            consistent &= t["checksum_consistent"]
         bestT = max(runs, key=runs.get)
         rec = {"planes": rule.planes, "gcups": runs[bestT], "temporal_steps": bestT,
                "gcups_by_depth": {f"T{T}": g for T, g in runs.items()},
+               "temporal_steps_used": {f"T{T}": u for T, u in used.items()},
+               "rel_std_by_depth": {f"T{T}": v for T, v in std.items()}, "checksum_consistent": consistent,
                "bit_exact_2048_subset": check_subset(C, O, name, "temporal", bestT),
                "reference_h100_best_gcups": H100_BEST.get(name)}
         print(json.dumps({name: rec}), file=sys.stderr, flush=True)
         res[name] = rec
-        del dev
+        del dev, init
         torch.cuda.empty_cache()
-    return {"multi_plane": {"workload": f"16384^2, seeded i.i.d., {a.multiplane_steps} generations per depth, "
-                                         "plain GPU path, linear bit planes", "automata": res}}
+    return {"multi_plane": {"workload": f"16384^2, seeded i.i.d., {a.multiplane_steps} generations per round, "
+                                         f"{a.rounds} rounds per depth, plain GPU path, linear bit planes",
+                            "automata": res}}
+
+
+# the reference's best published H100 GCUPS of its bit_array evaluator at 16384^2 x 128
+# (results/H100/_h100.csv, best configuration per word size)
+H100_BIT_ARRAY = {("game-of-life", 32): 285.8, ("game-of-life", 64): 225.6, ("wire", 32): 307.0, ("wire", 64): 285.9}
+
+
+def phase_bit_array(a):
+    """The bit_array layout (cells packed as P-bit indices, one generation per launch), a
+    comparison point: Game of Life and WireWorld at a.size^2 with 32- and 64-bit words, seeded
+    i.i.d., a.rounds rounds of a.bit_array_steps generations from the same initial state, words
+    equal across rounds; bit-exact on a 2048^2 subset against the standard evaluator."""
+    import numpy as np
+    import torch
+    import cellato_amd as C
+    import oracle_lib as O
+    from cellato_amd import _lib
+    res = {}
+    stream = torch.cuda.current_stream()
+    for name in ("game-of-life", "wire"):
+        rule = C.rule(name)
+        idx = rule.index_of(C.iid_values(name, a.size, a.size, 2026))
+        sub = C.iid_values(name, 2048, 2048, 11)
+        want = O.run_standard(name, sub, 64, threads=min(16, os.cpu_count() or 1))
+        for wb in (32, 64):
+            rows = 1024   # pack in bands of rows to bound the host memory
+            words = np.concatenate([C.grid.pack_bit_array(idx[r:r + rows], rule.planes, wb) for r in range(0, a.size, rows)])
+            init = torch.from_numpy(words.view(np.int64 if wb == 64 else np.int32)).to("cuda")
+            buf = [init.clone(), torch.zeros_like(init)]
+
+            def run(steps, t0=0):
+                return _lib.bit_array(rule.id, wb, buf[0].data_ptr(), buf[1].data_ptr(), a.size, a.size, t0, steps,
+                                      stream.cuda_stream)
+            run(4)
+            torch.cuda.synchronize()
+            times, finals = [], []
+            for _ in range(a.rounds):
+                buf[0].copy_(init)
+                torch.cuda.synchronize()
+                s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s0.record(stream)
+                in_b = run(a.bit_array_steps)
+                s1.record(stream)
+                s1.synchronize()
+                times.append(s0.elapsed_time(s1))
+                finals.append(buf[1 if in_b else 0].clone())
+            same = all(torch.equal(f, finals[0]) for f in finals)
+            del finals
+            ms = float(np.mean(times))
+            # bit-exact on the 2048^2 subset
+            sw = torch.from_numpy(C.grid.pack_bit_array(rule.index_of(sub), rule.planes, wb).view(
+                np.int64 if wb == 64 else np.int32)).to("cuda")
+            sb = torch.zeros_like(sw)
+            in_b = _lib.bit_array(rule.id, wb, sw.data_ptr(), sb.data_ptr(), 2048, 2048, 0, 64, stream.cuda_stream)
+            out = (sb if in_b else sw).cpu().numpy().view(np.uint64 if wb == 64 else np.uint32)
+            got = rule.value_of(C.grid.unpack_bit_array(out, 2048, 2048, rule.planes, wb))
+            rec = {"gcups": round(a.size * a.size * a.bit_array_steps / (ms * 1e-3) / 1e9, 1), "ms_mean": round(ms, 3),
+                   "ms_std": round(float(np.std(times)), 4), "rounds_had_same_result": bool(same),
+                   "bit_exact_2048_subset": bool(np.array_equal(got, want)),
+                   "reference_h100_best_gcups": H100_BIT_ARRAY.get((name, wb))}
+            res[f"{name} {wb}-bit"] = rec
+            print(json.dumps({name: {wb: rec}}), file=sys.stderr, flush=True)
+            del init, buf, sw, sb
+            torch.cuda.empty_cache()
+    return {"bit_array": {"workload": f"{a.size}^2, seeded i.i.d., {a.bit_array_steps} generations per round, "
+                                      f"{a.rounds} rounds, simple traverser, bit_array layout", "runs": res}}
 
 
 def phase_batch(a):
@@ -474,8 +652,8 @@ def phase_table(a):
     return {"table": out}
 
 
-PHASES = {"variants": phase_variants, "multiplane": phase_multiplane, "batch": phase_batch, "serve": phase_serve,
-          "cpu": phase_cpu, "table": phase_table}
+PHASES = {"variants": phase_variants, "multiplane": phase_multiplane, "bit_array": phase_bit_array,
+          "batch": phase_batch, "serve": phase_serve, "cpu": phase_cpu, "table": phase_table}
 
 
 def run_phase(name, argv):
@@ -518,8 +696,8 @@ def main():
                    "value": None, "unit": "GCUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                    "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                    "dtype": "u64", "data": "synthetic", "config": {"workload": "game-of-life 16384x16384"}}
-        for name, skip in (("multiplane", a.no_multiplane), ("batch", a.no_batch), ("serve", a.no_serve),
-                           ("cpu", a.no_cpu)):
+        for name, skip in (("multiplane", a.no_multiplane), ("bit_array", a.no_bit_array), ("batch", a.no_batch),
+                           ("serve", a.no_serve), ("cpu", a.no_cpu)):
             if skip:
                 continue
             res, err = run_phase(name, argv)
