@@ -23,20 +23,22 @@ sys.path.insert(0, str(REPO / "tests"))
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes and VALU instructions per launch measured with rocprofv3 (FETCH_SIZE / WRITE_SIZE
-# calibrated against a plain copy kernel, SQ_INSTS_VALU); see profiles/stream_kernel_16384_*.md
-TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_16384.json"
+# calibrated against a plain copy kernel, SQ_INSTS_VALU); see profiles/stream_kernel_hbm_traffic.md
+# and profiles/stream_kernel_16384_valu.md
+TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic.json"
 # VALU issue peak: one wave64 instruction per SIMD every 4 cycles, 256 CUs x 4 SIMDs, 2.4 GHz
 VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 
 
-def measured_launch(automaton, traverser, T, size, key="traffic_bytes"):
+def measured_launch(automaton, traverser, T, size, key="traffic_bytes", layout="linear"):
     try:
         data = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
     for e in data["entries"]:
-        if (e["automaton"], e["traverser"], e["temporal_steps"], e["size"]) == (automaton, traverser, T, size):
-            return int(e[key]) if key in e else None
+        if (e["automaton"], e["layout"], e["traverser"], e["temporal_steps"], e["size"]) == \
+                (automaton, layout, traverser, T, size):
+            return e[key] if key in e else None
     return None
 
 
@@ -328,11 +330,11 @@ def bench_serve(C, O, torch, tuner, n, rate, limit, tick_passes):
             "p99_vs_one_at_a_time": round(float(np.percentile(lat_all, 99)) / s["p99_s"], 2) if s["p99_s"] else None}
 
 
-def valu_roofline(g, size, ms_per_launch):
+def valu_roofline(g, size, ms_per_launch, layout="linear"):
     """Temporal launches are bound by VALU issue, not HBM: wave64 VALU instructions per launch
     (rocprofv3 SQ_INSTS_VALU, a fixed count for the configuration) over the live average
     launch duration, against the issue peak."""
-    ins = measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], size, "valu_wave_instructions")
+    ins = measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], size, "valu_wave_instructions", layout)
     if not ins:
         return None
     rate = ins / (ms_per_launch * 1e-3)
@@ -394,7 +396,7 @@ def phase_variants(a):
     rule, dev = grids[g["automaton"]]
     bytes_per_launch = 2 * rule.planes * a.size * a.size // 8
     ms_launch, achieved = g["us_per_launch_b2b"] * 1e-3, g["algorithmic_GBps_b2b"]
-    valu = valu_roofline(g, a.size, ms_launch)
+    valu = valu_roofline(g, a.size, ms_launch, a.layout)
     simple_gol = next((r for r in variants if r["automaton"] == g["automaton"] and r["traverser"] == "simple"), None)
     simple_roof = None
     if simple_gol is not None:
@@ -403,7 +405,7 @@ def phase_variants(a):
                        "frac": round(gbps_s / HBM_PEAK_GBPS, 4), "kernel": "stream_kernel (simple)",
                        "us_per_launch": simple_gol["us_per_launch_b2b"], "launches_timed": a.roofline_launches,
                        "bytes_per_launch": bytes_per_launch,
-                       "traffic": measured_launch(g["automaton"], "simple", 1, a.size)}
+                       "traffic": measured_launch(g["automaton"], "simple", 1, a.size, layout=a.layout)}
     for name, (r, d) in grids.items():
         del d
     grids.clear()
@@ -455,7 +457,8 @@ def phase_variants(a):
         "checksum_consistent": all(r["checksum_consistent"] for r in variants),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], a.size),
+                     "traffic": measured_launch(g["automaton"], g["traverser"], g["temporal_steps"], a.size,
+                                                layout=a.layout),
                      "kernel": f"stream_kernel ({g['traverser']}, T={g['temporal_steps']})",
                      "bytes_per_launch": bytes_per_launch,
                      "us_per_launch": round(ms_launch * 1e3, 3), "launches_timed": a.roofline_launches,

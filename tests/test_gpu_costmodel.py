@@ -67,10 +67,13 @@ def test_network_beats_tuning_without_it():
     assert by_size["model_topk"] >= 0.95
 
 
-def test_unseen_automaton_is_reported():
-    # With nine automata to learn from, the network does not transfer to a tenth it never
-    # saw (measured: below random picks); the evaluation runs and is reported, not claimed.
+def test_network_transfers_to_an_unseen_automaton():
+    # Trained on nine automata, the network ranks the launch shapes of the tenth (never seen)
+    # from what its rule is made of and what the compiler made of it (planes, AST node kinds,
+    # registers per lane, waves per SIMD): its two picks land closer to the best than two
+    # random picks or the fixed depth T=4.
     from cellato_amd.tuning_eval import evaluate
     by_rule = evaluate(budget=2, epochs=600, holdout="rule")
     print(by_rule)
-    assert 0.0 < by_rule["model_topk"] <= 1.0
+    assert by_rule["model_topk"] > by_rule["random_k"] + 0.01
+    assert by_rule["model_topk"] > by_rule["default_T4"] + 0.01
