@@ -593,7 +593,7 @@ def phase_batch(a):
     tuner, train_s = make_tuner(a.tuner_cache)
     gains = {h: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in evaluate(budget=2, holdout=h).items()}
              for h in ("size", "automaton")}
-    out = {"tuner": {"model": "MLP 15-64-64-1 trained on the GPU (csrc/mlp.hip) on "
+    out = {"tuner": {"model": f"MLP {'-'.join(str(d) for d in tuner.model.dims)} trained on the GPU (csrc/mlp.hip) on "
                               "cellato_amd/data/depth_tuning_mi355x.json", "train_seconds": round(train_s, 2),
                      "measured_per_bucket": tuner.budget, "held_out_fraction_of_best": gains}}
     out["batched"] = bench_batch(C, O, torch, tuner, a.batch_grids)
