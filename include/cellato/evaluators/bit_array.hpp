@@ -16,7 +16,7 @@
 
 #include "../core/ast.hpp"
 
-#if defined(__HIPCC__) || defined(__CUDACC__)
+#if defined(__HIPCC__)
 #define CELLATO_BA_HD __host__ __device__ __forceinline__
 #else
 #define CELLATO_BA_HD inline
