@@ -16,7 +16,10 @@ pytestmark = pytest.mark.gpu
 
 RULES = ["game-of-life", "wire", "forest-fire", "excitable", "brian", "maze", "traffic", "fluid", "critters", "cyclic"]
 LINEAR_SHAPES = [(192, 70), (4096, 40), (8192, 50), (8320, 33), (4160, 20), (16384, 24), (7936, 12), (128, 1), (64, 3)]
-TILED_SHAPES = [(40, 24), (512, 16), (1024, 24), (1040, 32), (520, 40), (8, 8)]
+# 4096: one ring of 64 block columns x 8; 4160, 8320, 4104: segments of 62 blocks and a tail of
+# narrow lane groups
+TILED_SHAPES = [(40, 24), (512, 16), (1024, 24), (1040, 32), (520, 40), (8, 8), (4096, 40), (4160, 16), (8320, 24),
+                (4104, 16)]
 TRAVERSERS = [("simple", 1), ("temporal", 1), ("temporal", 2), ("temporal", 3), ("temporal", 5), ("temporal", 8)]
 
 
@@ -46,6 +49,12 @@ def test_chunk_sizes(rule):
     for chunk in (1, 2, 3, 7, 64):
         for kind, T in [("simple", 1), ("temporal", 4), ("temporal", 8)]:
             assert np.array_equal(run_gpu(rule, v, 0, kind, T, 16, 1, chunk), want), (chunk, kind, T)
+    # the tiled layout: chunks of block rows
+    v = iid(rule, 8320, 48, 10)
+    want = O.run_standard(rule, v, 16, t0=1)
+    for chunk in (1, 2, 5):
+        for T in (2, 4, 8):
+            assert np.array_equal(run_gpu(rule, v, 1, "temporal", T, 16, 1, chunk), want), ("tiled", chunk, T)
 
 
 @pytest.mark.parametrize("rule", RULES)
