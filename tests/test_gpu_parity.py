@@ -111,3 +111,41 @@ def test_large_grid_paths_agree(rule):
     assert bool(results[0].any()), f"{rule}: empty grid after {steps} generations"
     assert torch.equal(results[0], results[1]), f"{rule}: simple != temporal T=3 at {n}^2"
     assert torch.equal(results[0], results[2]), f"{rule}: simple != temporal T=8 at {n}^2"
+
+
+@pytest.mark.parametrize("layout", [0, 1], ids=["linear", "tiled"])
+def test_largest_plane_paths_agree(layout):
+    """A plane of 4 GiB - 512 KiB, the largest the 32-bit buffer offsets address (Game of Life
+    262144 x 131056 cells; offsets past 2^31, rows of 4096 words): the simple traverser and
+    the temporal one at depths 4 and 8 give the same grid after 8 generations from the same
+    device-made state (16 GiB of device memory for the two grids being compared)."""
+    import torch
+    import cellato_amd as C
+    from cellato_amd import _lib
+    from cellato_amd.planes import DeviceGrid
+    r = C.rule("game-of-life")
+    W, H, steps = 262144, 131056, 8
+    assert W * H // 8 == 2**32 - 2**19
+    stream = torch.cuda.current_stream().cuda_stream
+    ref = None
+    for kind, T in (("simple", 1), ("temporal", 4), ("temporal", 8)):
+        print(f"[gpu] largest plane layout={layout} {kind} T={T}", flush=True)
+        dev = DeviceGrid(r, W, H, layout)
+        dev.fill_random(9)
+        if kind == "simple":
+            in_b = _lib.simple(r.id, layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, 0, steps, 0, stream)
+        else:
+            in_b = _lib.temporal(r.id, layout, dev.ptrs(dev.cur), dev.ptrs(1 - dev.cur), W, H, 0, steps, T, 0, stream)
+        if in_b:
+            dev.cur = 1 - dev.cur
+        torch.cuda.synchronize()
+        out = dev.buf[dev.cur]
+        if ref is None:
+            ref = out.clone()
+            assert bool(ref.any()), "empty grid"
+        else:
+            assert torch.equal(ref, out), f"layout {layout}: simple != temporal T={T}"
+        del dev, out
+        torch.cuda.empty_cache()
+    del ref
+    torch.cuda.empty_cache()
