@@ -464,7 +464,11 @@ def phase_variants(a):
                      "us_per_launch": round(ms_launch * 1e3, 3), "launches_timed": a.roofline_launches,
                      # at T > 1 generations per launch the kernel is limited by VALU issue, not HBM
                      "limiter": "valu" if g["temporal_steps"] > 1 and g["traverser"] == "temporal" else "hbm",
-                     "valu": valu},
+                     "valu": valu,
+                     # the bytes a one-generation-per-launch traverser would move for the same cell
+                     # updates (P/4 per update), over the same time: what temporal blocking gains
+                     "single_step_equivalent_GBps": round(g["gcups"] * rule.planes / 4, 1),
+                     "single_step_equivalent_frac": round(g["gcups"] * rule.planes / 4 / HBM_PEAK_GBPS, 4)},
         "roofline_simple": simple_roof,
         "variants": variants,
     }
