@@ -897,6 +897,229 @@ CELLATO_HD word cc_eval(const Win& w) {
     }(std::make_integer_sequence<unsigned, 16>{});
 }
 
+// ---- rules of the cell and neighbourhood predicates (any plane count) ----------------------
+// A rule whose neighbourhood enters only through boolean predicates that do not read the cell
+// itself (comparisons of a state's Moore or von Neumann count with a constant, and and / or /
+// not of those), around logic on the cell's own state (WireWorld, Brian's Brain, Greenberg-
+// Hastings, forest fire: an if / elif chain over the cell's state, one predicate), is a table
+// of the cell's dictionary index and the predicate bits. The predicates are evaluated as
+// usual (from row sums where the window has them); the table is evaluated at compile time and
+// each plane of the result becomes a boolean function of the P cell planes and the M predicate
+// masks, built from three-input ops by Shannon expansion (3 inputs: one op; 4: three), cell
+// indices past the dictionary's states left free. Greenberg-Hastings' eight-way chain becomes
+// three ops per plane instead of a compare and a 3-plane select per branch.
+template <class... T>
+struct tlist {
+    static constexpr int size = sizeof...(T);
+};
+template <class L, class X>
+struct tl_add;
+template <class... T, class X>
+struct tl_add<tlist<T...>, X> {
+    using type = std::conditional_t<(std::is_same_v<T, X> || ...), tlist<T...>, tlist<T..., X>>;
+};
+template <class L, class X>
+struct tl_index;
+template <class X>
+struct tl_index<tlist<>, X> { static constexpr int value = -1; };
+template <class H, class... T, class X>
+struct tl_index<tlist<H, T...>, X> {
+    static constexpr int value = std::is_same_v<H, X> ? 0 : (tl_index<tlist<T...>, X>::value < 0 ? -1 : 1 + tl_index<tlist<T...>, X>::value);
+};
+
+// pt_pred: a boolean node that reads the neighbourhood and not the cell
+template <class Dict, class E>
+struct pt_pred : std::false_type {};
+template <class N>
+struct pt_nbhd : std::false_type {};
+template <> struct pt_nbhd<moore_8_neighbors> : std::true_type {};
+template <> struct pt_nbhd<von_neumann_4_neighbors> : std::true_type {};
+#define CELLATO_PT_CMP(node)                                                                    \
+    template <class Dict, auto V, class N, auto K>                                              \
+    struct pt_pred<Dict, node<count_neighbors<state_constant<V>, N>, constant<K>>> : pt_nbhd<N> {};
+CELLATO_PT_CMP(equals)
+CELLATO_PT_CMP(not_equals)
+CELLATO_PT_CMP(greater_than)
+CELLATO_PT_CMP(less_than)
+#undef CELLATO_PT_CMP
+template <class Dict, class L, class R>
+struct pt_pred<Dict, and_<L, R>> : std::bool_constant<pt_pred<Dict, L>::value && pt_pred<Dict, R>::value> {};
+template <class Dict, class L, class R>
+struct pt_pred<Dict, or_<L, R>> : std::bool_constant<pt_pred<Dict, L>::value && pt_pred<Dict, R>::value> {};
+template <class Dict, class V>
+struct pt_pred<Dict, not_<V>> : pt_pred<Dict, V> {};
+
+// pt_expr<Dict, E, Preds>: whether E is of the form above (with its maximal predicates in
+// Preds), and its value for a cell index and predicate bits (bit i = predicate i of Preds)
+template <class Dict, class E, class Preds, bool Pred = pt_pred<Dict, E>::value>
+struct pt_expr {   // not of that form
+    static constexpr bool ok = false;
+    using preds = Preds;
+    static constexpr long long at(long long, unsigned) { return 0; }
+};
+template <class Dict, class E, class Preds>
+struct pt_expr<Dict, E, Preds, true> {   // a predicate: a leaf
+    static constexpr bool ok = true;
+    using preds = typename tl_add<Preds, E>::type;
+    static constexpr long long at(long long, unsigned bits) { return (bits >> tl_index<preds, E>::value) & 1u; }
+};
+template <class Dict, auto V, class Preds>
+struct pt_expr<Dict, constant<V>, Preds, false> {
+    static constexpr bool ok = true;
+    using preds = Preds;
+    static constexpr long long at(long long, unsigned) { return static_cast<long long>(V); }
+};
+template <class Dict, auto V, class Preds>
+struct pt_expr<Dict, state_constant<V>, Preds, false> {
+    static constexpr bool ok = true;
+    using preds = Preds;
+    static constexpr long long at(long long, unsigned) { return index_of<Dict, V>; }
+};
+template <class Dict, class Preds>
+struct pt_expr<Dict, neighbor_at<0, 0>, Preds, false> {
+    static constexpr bool ok = true;
+    using preds = Preds;
+    static constexpr long long at(long long cell, unsigned) { return cell; }
+};
+// interior nodes: the operands in order, each seeing the predicates collected so far; `at`
+// evaluates against the final list
+template <class Dict, class Preds, class... A>
+struct pt_chain;
+template <class Dict, class Preds>
+struct pt_chain<Dict, Preds> {
+    static constexpr bool ok = true;
+    using preds = Preds;
+};
+template <class Dict, class Preds, class H, class... T>
+struct pt_chain<Dict, Preds, H, T...> {
+    using head = pt_expr<Dict, H, Preds>;
+    using tail = pt_chain<Dict, typename head::preds, T...>;
+    static constexpr bool ok = head::ok && tail::ok;
+    using preds = typename tail::preds;
+};
+template <class Dict, class E, class Final>
+struct pt_at {   // E evaluated with the predicate positions of the final list
+    static constexpr long long at(long long c, unsigned b) { return pt_expr<Dict, E, Final>::at(c, b); }
+};
+#define CELLATO_PT_BINARY(node, op)                                                             \
+    template <class Dict, class L, class R, class Preds>                                        \
+    struct pt_expr<Dict, node<L, R>, Preds, false> : pt_chain<Dict, Preds, L, R> {              \
+        using final_preds = typename pt_chain<Dict, Preds, L, R>::preds;                        \
+        static constexpr long long at(long long c, unsigned b) {                                \
+            return pt_at<Dict, L, final_preds>::at(c, b) op pt_at<Dict, R, final_preds>::at(c, b) ? 1 : 0; \
+        }                                                                                       \
+    };
+CELLATO_PT_BINARY(equals, ==)
+CELLATO_PT_BINARY(not_equals, !=)
+CELLATO_PT_BINARY(less_than, <)
+CELLATO_PT_BINARY(greater_than, >)
+CELLATO_PT_BINARY(and_, &&)
+CELLATO_PT_BINARY(or_, ||)
+#undef CELLATO_PT_BINARY
+template <class Dict, class V, class Preds>
+struct pt_expr<Dict, not_<V>, Preds, false> : pt_chain<Dict, Preds, V> {
+    using final_preds = typename pt_chain<Dict, Preds, V>::preds;
+    static constexpr long long at(long long c, unsigned b) { return pt_at<Dict, V, final_preds>::at(c, b) ? 0 : 1; }
+};
+template <class Dict, class C, class A, class B, class Preds>
+struct pt_expr<Dict, if_then_else<C, A, B>, Preds, false> : pt_chain<Dict, Preds, C, A, B> {
+    using final_preds = typename pt_chain<Dict, Preds, C, A, B>::preds;
+    static constexpr long long at(long long c, unsigned b) {
+        return pt_at<Dict, C, final_preds>::at(c, b) ? pt_at<Dict, A, final_preds>::at(c, b) : pt_at<Dict, B, final_preds>::at(c, b);
+    }
+};
+
+// comparisons of the cell with a state in E (`current_state == S`)
+template <class E>
+struct pt_cell_tests { static constexpr int value = 0; };
+template <template <class...> class N, class... A>
+struct pt_cell_tests<N<A...>> { static constexpr int value = (0 + ... + pt_cell_tests<A>::value); };
+template <auto V>
+struct pt_cell_tests<equals<neighbor_at<0, 0>, state_constant<V>>> { static constexpr int value = 1; };
+
+// Taken for rules that compare the cell with three states or more: shorter chains the compiler
+// already folds into the cell tests (measured VALU per launch at 16384^2, table / node by node:
+// Greenberg-Hastings 0.55-0.57, forest fire 0.68-0.69, WireWorld 0.95-0.96, Brian's Brain with
+// two comparisons 1.12).
+template <class Dict, class E, int P>
+struct pt_rule {
+    using X = pt_expr<Dict, E, tlist<>>;
+    using preds = typename X::preds;
+    static constexpr int M = preds::size;
+    static constexpr int n = P + M;   // inputs: the cell's planes, then the predicates
+    static constexpr bool ok = X::ok && M >= 1 && n <= 6 && pt_cell_tests<E>::value >= 3;
+    // truth table of result plane `b` (bit i: inputs i, cell index in the low P bits) and the
+    // inputs that do not occur (cell indices past the dictionary)
+    static constexpr unsigned long long table(int b) {
+        unsigned long long t = 0;
+        for (int i = 0; i < (1 << n); ++i) {
+            const long long cell = i & ((1 << P) - 1);
+            if (cell < Dict::number_of_values && ((pt_at<Dict, E, preds>::at(cell, static_cast<unsigned>(i >> P)) >> b) & 1))
+                t |= 1ull << i;
+        }
+        return t;
+    }
+    static constexpr unsigned long long free_inputs() {
+        unsigned long long f = 0;
+        for (int i = 0; i < (1 << n); ++i)
+            if ((i & ((1 << P) - 1)) >= Dict::number_of_values) f |= 1ull << i;
+        return f;
+    }
+};
+
+// A boolean function of the first N inputs in[0..N-1] (truth table F, free entries D) from
+// three-input ops: one op for N <= 3, else a select on the last input between the two halves.
+template <int N, unsigned long long F, unsigned long long D>
+CELLATO_HD word pt_fn(const word* in) {
+    constexpr unsigned long long all = N >= 6 ? ~0ull : ((1ull << (1 << N)) - 1);
+    if constexpr (((F & ~D) & all) == 0) {
+        return word(0);
+    } else if constexpr (((F | D) & all) == all) {
+        return ~word(0);
+    } else if constexpr (N <= 3) {
+        // free entries take 0; the LUT over (in[2], in[1], in[0]) = index 4 a + 2 b + c
+        constexpr unsigned lut = [] {
+            unsigned l = 0;
+            for (int i = 0; i < 8; ++i) {
+                const int j = i & ((1 << N) - 1);   // inputs past N do not matter
+                if ((F >> j) & 1 & ~((D >> j) & 1)) l |= 1u << i;
+            }
+            return l;
+        }();
+        const word a = N >= 3 ? in[2] : word(0), b = N >= 2 ? in[1] : word(0), c = in[0];
+        return lop3<lut>(a, b, c);
+    } else {
+        constexpr int h = 1 << (N - 1);
+        constexpr unsigned long long lo_mask = (1ull << h) - 1;
+        constexpr unsigned long long F0 = F & lo_mask, F1 = (F >> h) & lo_mask;
+        constexpr unsigned long long D0 = D & lo_mask, D1 = (D >> h) & lo_mask;
+        if constexpr (((F0 ^ F1) & ~(D0 | D1) & lo_mask) == 0) {
+            // the last input does not matter (entries free in one half follow the other)
+            return pt_fn<N - 1, (F0 & ~D0) | (F1 & ~D1), D0 & D1>(in);
+        } else {
+            const word f1 = pt_fn<N - 1, F1, D1>(in);
+            const word f0 = pt_fn<N - 1, F0, D0>(in);
+            return lop3<0xCA>(in[N - 1], f1, f0);   // in[N-1] ? f1 : f0
+        }
+    }
+}
+
+template <class Dict, class E, int P, class Win, class... Pr>
+CELLATO_HD vint<P> pt_eval(const Win& w, tlist<Pr...>) {
+    using R = pt_rule<Dict, E, P>;
+    word in[P + sizeof...(Pr)];
+    const vint<P> cell = w.template nb<0, 0>();
+#pragma unroll
+    for (int i = 0; i < P; ++i) in[i] = cell.p[i];
+    int k = P;
+    ((in[k++] = ev<Dict, Pr>::at(w)), ...);
+    vint<P> r;
+    [&]<int... B>(std::integer_sequence<int, B...>) {
+        ((r.p[B] = pt_fn<R::n, R::table(B), R::free_inputs()>(in)), ...);
+    }(std::make_integer_sequence<int, P>{});
+    return r;
+}
+
 // The reference's evaluator name: evaluator<word, dictionary, Expression>
 // (evaluators/bit_planes.hpp:48-49). Only 64-bit words are implemented on the GPU.
 template <typename cell_row_type, typename state_dictionary_type, typename Expression>
@@ -921,6 +1144,9 @@ struct evaluator {
             vint<1> r;
             r.p[0] = cc_eval<state_dictionary_type, Expression>(w);
             return r;
+        } else if constexpr (planes > 1 && pt_rule<state_dictionary_type, Expression, planes>::ok) {
+            return pt_eval<state_dictionary_type, Expression, planes>(
+                w, typename pt_rule<state_dictionary_type, Expression, planes>::preds{});
         } else {
             return resize<planes>(as_vint(eval<state_dictionary_type, Expression>(w)));
         }

@@ -23,6 +23,19 @@ constexpr bool table_path = cellato::automata::by_id<id>::config::state_dictiona
 static_assert(table_path<cellato::automata::GAME_OF_LIFE> && table_path<cellato::automata::MAZE>);
 static_assert(!table_path<cellato::automata::WIRE> && !table_path<cellato::automata::CRITTERS> &&
               !table_path<cellato::automata::BRIAN> && !table_path<cellato::automata::CYCLIC>);
+// Rules of several planes whose neighbourhood enters only through predicates of neighbour
+// counts, and that compare the cell with three states or more, take the predicate-table path
+// (every window); the others do not.
+template <int id>
+constexpr bool predicate_path = cellato::automata::by_id<id>::config::state_dictionary::needed_bits > 1 &&
+    bp::pt_rule<typename cellato::automata::by_id<id>::config::state_dictionary,
+                typename cellato::automata::by_id<id>::config::algorithm,
+                cellato::automata::by_id<id>::config::state_dictionary::needed_bits>::ok;
+static_assert(predicate_path<cellato::automata::WIRE> && predicate_path<cellato::automata::EXCITABLE> &&
+              predicate_path<cellato::automata::FOREST_FIRE>);
+static_assert(!predicate_path<cellato::automata::TRAFFIC> && !predicate_path<cellato::automata::FLUID> &&
+              !predicate_path<cellato::automata::CYCLIC> && !predicate_path<cellato::automata::GAME_OF_LIFE> &&
+              !predicate_path<cellato::automata::BRIAN>);   // two cell comparisons: node by node
 
 using namespace cellato::evaluators::bit_planes;
 using u64 = std::uint64_t;

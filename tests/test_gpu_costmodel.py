@@ -60,11 +60,11 @@ def test_network_beats_tuning_without_it():
     by_size = evaluate(budget=2, epochs=600, holdout="size")
     print(by_size)
     # Two measurements guided by the network, on grid sizes the network never saw, land
-    # closer to the best depth than two random measurements or a fixed depth, and within 5%
-    # of measuring all seven depths.
-    assert by_size["model_topk"] > by_size["random_k"] + 0.02
-    assert by_size["model_topk"] > by_size["default_T4"]
-    assert by_size["model_topk"] >= 0.95
+    # clearly closer to the best launch shape than two random measurements or the fixed shape
+    # T=4, and within 7 % of measuring all 21 shapes.
+    assert by_size["model_topk"] > by_size["random_k"] + 0.05
+    assert by_size["model_topk"] > by_size["default_T4"] + 0.03
+    assert by_size["model_topk"] >= 0.93
 
 
 def test_network_transfers_to_an_unseen_automaton():
