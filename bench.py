@@ -486,7 +486,7 @@ H100_BEST = {"cyclic": 1505, "excitable": 3146, "fluid": 9197, "critters": 4680}
 def phase_multiplane(a):
     """Multi-plane rules on the plain GPU path: Cyclic (5 planes), Greenberg-Hastings (3),
     HPP fluid (4), Critters (1, Margolus), 16384^2, seeded i.i.d., temporal traverser at
-    depths 2/4/8, each timed as warm-up plus a.rounds rounds from the same initial state; per
+    depths 2/4/6/8, each timed as warm-up plus a.rounds rounds from the same initial state; per
     automaton the best depth, re-run bit-exact on a 2048^2 subset against the standard
     evaluator. A depth whose kernel would spill registers runs at the deepest one that does
     not (`temporal_steps_used`)."""
@@ -501,7 +501,7 @@ def phase_multiplane(a):
         dev.upload(C.grid.pack(rule.index_of(C.iid_values(name, a.size, a.size, 2026)), rule.planes, 0))
         init = dev.buf[0].clone()
         runs, used, std, consistent = {}, {}, {}, True
-        for T in (2, 4, 8):
+        for T in (2, 4, 6, 8):
             t = time_variant(C, torch, rule, dev, init, "temporal", T, a.multiplane_steps, 10, a.rounds, 0)
             runs[T] = round(t["gcups"], 1)
             std[T] = round(t["ms_std"] / t["ms_mean"], 4)
