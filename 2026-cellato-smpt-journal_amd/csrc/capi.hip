@@ -166,7 +166,7 @@ int cellato_hip_plan(int rule, int layout, int width, int height, int temporal_s
     const int T = temporal ? usable_depth(rule, layout, words_per_row(layout, width), temporal_steps) : 1;
     const int rc = plan_pass(rule, layout, width, height, T, temporal, chunk_rows, pl);
     if (rc != CELLATO_OK) return fail(rc, plan_error());
-    if (chunk) *chunk = pl.args.chunk;
+    if (chunk) *chunk = pl.rows ? pl.args.chunk / 8 : pl.args.chunk;   // in rows of the layout's words
     if (tasks) *tasks = pl.args.ntasks;
     if (blocks) *blocks = pl.blocks;
     return CELLATO_OK;

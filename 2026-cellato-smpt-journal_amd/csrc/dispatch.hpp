@@ -38,6 +38,7 @@ struct rule_kernels {
     int state_values[32];        // enum value of each dictionary index (for checksums)
     const void* simple[2][SIMPLE_VARIANTS];                   // [layout][variant]
     const void* temporal[2][TEMPORAL_VARIANTS][MAX_T + 1];    // [layout][variant][T], T >= 1
+    const void* temporal_rows[2][MAX_T + 1];                  // tiled planes processed as rows: [segments / ring of 64][T], T >= 2
     const void* batch[MAX_T + 1];                             // linear layout, [T], T >= 1
     const void* bit_array[2];                                 // bit_array layout, [0] 32-bit, [1] 64-bit words
 };
@@ -64,11 +65,30 @@ constexpr void fill_layout(rule_kernels& k) {
     }(std::make_integer_sequence<int, MaxT>{});
 }
 
+// Temporal kernels that process tiled planes as rows (TILED_ROWS), for depths FromT .. MaxT of
+// rules of at most two planes (the LDS staging areas take 32 KiB per plane and block); null
+// elsewhere. The planner takes them for rows of at least 64 columns of 64 cells.
+template <class Eval, int FromT, int MaxT>
+constexpr void fill_rows(rule_kernels& k) {
+    for (int v = 0; v < 2; ++v)
+        for (int t = 0; t <= MAX_T; ++t) k.temporal_rows[v][t] = nullptr;
+    if constexpr (Eval::planes <= 2 && FromT >= 2 && MaxT >= FromT) {
+        [&]<int... Ts>(std::integer_sequence<int, Ts...>) {
+            ((k.temporal_rows[0][Ts + FromT] =
+                  reinterpret_cast<const void*>(&stream_kernel<Eval, TILED_ROWS, Ts + FromT, XCH_SEGMENT, HALO_LANES>)), ...);
+            ((k.temporal_rows[1][Ts + FromT] =
+                  reinterpret_cast<const void*>(&stream_kernel<Eval, TILED_ROWS, Ts + FromT, XCH_RING64, HALO_NONE>)), ...);
+        }(std::make_integer_sequence<int, MaxT - FromT + 1>{});
+    }
+}
+
 // MaxLinearT / MaxTiledT: the deepest depth whose kernels of that layout fit the register file
 // without scratch memory (every ring slot also holds the neighbours' edge bits: halves of the
 // neighbouring words in the linear layout, an edge word per plane in the tiled one, so a rule
-// with many planes runs out of VGPRs at a lower depth).
-template <class Config, int MaxLinearT = MAX_T, int MaxTiledT = MAX_T>
+// with many planes runs out of VGPRs at a lower depth). RowsFromT: the shallowest depth from
+// which the tiled layout's temporal passes run as rows (TILED_ROWS), measured per automaton
+// at 16384^2 as the depth from which they beat the block kernels; above MAX_T: never.
+template <class Config, int MaxLinearT = MAX_T, int MaxTiledT = MAX_T, int RowsFromT = MAX_T + 1>
 rule_kernels make_rule_kernels() {
     using Eval = cellato::evaluators::bit_planes::evaluator<word, typename Config::state_dictionary, typename Config::algorithm>;
     rule_kernels k{};
@@ -80,6 +100,7 @@ rule_kernels make_rule_kernels() {
         k.state_values[i] = static_cast<int>(Config::state_dictionary::index_to_state(i));
     fill_layout<Eval, LINEAR, MaxLinearT>(k);
     fill_layout<Eval, TILED, MaxTiledT>(k);
+    fill_rows<Eval, RowsFromT, MaxLinearT>(k);
     k.bit_array[0] = reinterpret_cast<const void*>(
         &bit_array_kernel<typename Config::state_dictionary, typename Config::algorithm, std::uint32_t>);
     k.bit_array[1] = reinterpret_cast<const void*>(

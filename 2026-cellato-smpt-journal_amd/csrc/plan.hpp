@@ -19,6 +19,7 @@ struct pass_plan {
     pass_args args{};
     int blocks = 0;
     long long streams = 0;   // rows x lane groups of work (the unit auto_chunk divides)
+    bool rows = false;       // tiled planes processed as rows: args count 64-cell columns and cell rows
 };
 
 inline std::string& plan_error() {
@@ -47,8 +48,8 @@ inline int device_cus() {
     return cus;
 }
 
-// Blocks of a kernel (BLOCK_THREADS threads, no LDS) one CU holds at once, from the
-// kernel's register use; cached per kernel. 8 when it cannot be asked (no device).
+// Blocks of a kernel (BLOCK_THREADS threads) one CU holds at once, from the kernel's register
+// and static LDS use; cached per kernel. 8 when it cannot be asked (no device).
 inline int blocks_per_cu(const void* fn) {
     static std::mutex mu;
     static std::map<const void*, int> cache;
@@ -84,9 +85,19 @@ inline int private_bytes(const void* fn) {
 // words per row of a grid `width` cells wide (linear: 64 cells per word; tiled: 8)
 inline int words_per_row(int layout, int width) { return layout == CELLATO_LAYOUT_LINEAR ? width / 64 : width / 8; }
 
+// Whether a temporal pass of depth T over tiled planes of wx block words per row (and h block
+// rows, if known) runs as rows (TILED_ROWS): a row of at least 64 columns of 64 cells, a rule
+// with those kernels, and planes within the bound of its 16-byte out-of-range accesses.
+inline bool rows_pass(const rule_kernels& k, int layout, int wx, int T, bool temporal, long long h = -1) {
+    if (!temporal || layout != CELLATO_LAYOUT_TILED || T < 2 || wx % 8 != 0 || wx / 8 < 64) return false;
+    if (h >= 0 && static_cast<unsigned long long>(wx) * static_cast<unsigned long long>(h) * 8ULL > OOB_WIDE) return false;
+    return k.temporal_rows[wx / 8 == 64 ? 1 : 0][T] != nullptr;
+}
+
 // Kernel of one pass: the stream kernel of the lane mapping that a row of wx words takes.
 inline const void* pass_kernel(const rule_kernels& k, int layout, int wx, int T, bool temporal) {
     if (!temporal) return k.simple[layout][wx < 64 ? SIMPLE_GROUPS : wx == 64 ? SIMPLE_RING64 : SIMPLE_SEGMENT];
+    if (rows_pass(k, layout, wx, T, temporal)) return k.temporal_rows[wx / 8 == 64 ? 1 : 0][T];
     return k.temporal[layout][wx < 64 ? TEMPORAL_GROUPS : wx == 64 ? TEMPORAL_RING64 : TEMPORAL_SEGMENT][T];
 }
 
@@ -135,12 +146,17 @@ inline int auto_chunk(TasksOf tasks_of, int T, int rows_per_group, int resident 
     return static_cast<int>(best);
 }
 
-// width / height in cells. T = generations per launch (1 for the simple traverser).
+// width / height in cells. T = generations per launch (1 for the simple traverser). chunk:
+// rows of words per task (tiled: block rows), 0 = auto_chunk.
 inline int plan_pass(int rule, int layout, int width, int height, int T, bool temporal, int chunk, pass_plan& pl,
                      long long launch_streams = 0) {
     const rule_kernels& k = kernels_of_rule(rule);
-    const int wx = layout == CELLATO_LAYOUT_LINEAR ? width / 64 : width / 8;
-    const int h = layout == CELLATO_LAYOUT_LINEAR ? height : height / 8;
+    // tiled planes processed as rows take the linear geometry: 64-cell columns, cell rows
+    pl.rows = rows_pass(k, layout, width / 8, T, temporal, height / 8);
+    const bool lin = layout == CELLATO_LAYOUT_LINEAR || pl.rows;
+    const int wx = lin ? width / 64 : width / 8;
+    const int h = lin ? height : height / 8;
+    if (pl.rows && chunk > 0) chunk = chunk < (1 << 27) ? 8 * chunk : (1 << 30);
     pass_args& a = pl.args;
     if (static_cast<unsigned long long>(wx) * static_cast<unsigned long long>(h) * 8ULL >= (1ULL << 32)) {
         plan_error() = "a bit plane must be smaller than 4 GiB";
@@ -162,7 +178,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
         a.stride = 0;
         a.nseg = 1;
         a.variant = VARIANT_RING64;
-        pl.fn = temporal ? k.temporal[layout][TEMPORAL_RING64][T] : k.simple[layout][SIMPLE_RING64];
+        pl.fn = pl.rows ? k.temporal_rows[1][T] : temporal ? k.temporal[layout][TEMPORAL_RING64][T] : k.simple[layout][SIMPLE_RING64];
     } else if (!temporal) {
         a.n = 64;
         a.groups = 1;
@@ -185,7 +201,7 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
             a.tail_first = 62 * a.nseg - 1;
         }
         a.variant = VARIANT_SEGMENT;
-        pl.fn = k.temporal[layout][TEMPORAL_SEGMENT][T];
+        pl.fn = pl.rows ? k.temporal_rows[0][T] : k.temporal[layout][TEMPORAL_SEGMENT][T];
     }
     if (!pl.fn) {
         plan_error() = "no kernel instantiated for this configuration";
@@ -205,6 +221,8 @@ inline int plan_pass(int rule, int layout, int width, int height, int T, bool te
     // every wave walk the wrapped rows again)
     a.chunk = chunk > 0 ? (chunk < rows_per_group ? chunk : rows_per_group)
                         : auto_chunk(tasks_of, T, rows_per_group, blocks_per_cu(pl.fn));
+    // rows: tasks cover whole block rows (h is a multiple of 8)
+    if (pl.rows) a.chunk = (a.chunk + 7) / 8 * 8 < h ? (a.chunk + 7) / 8 * 8 : h;
     const long long nchunk = (h + static_cast<long long>(a.groups) * a.chunk - 1) / (static_cast<long long>(a.groups) * a.chunk);
     const long long main_tasks = static_cast<long long>(a.nseg) * nchunk;
     const long long tasks = main_tasks + (a.tail_n > 0 ? (h + static_cast<long long>(a.tail_groups) * a.chunk - 1) /

@@ -9,7 +9,8 @@
 // shifts (full 64-lane segments) or ds_bpermute (rings of fewer lanes); the lanes at the
 // ends of a segment either load their outside neighbour (one generation per launch) or are
 // halo lanes whose results are discarded (T generations per launch, segments overlap by
-// two words). Nothing is staged in LDS; there is no barrier in the loop.
+// two words). Nothing is shared through LDS and there is no barrier in the loop (the tiled
+// planes processed as rows stage each lane's own 8-row blocks in LDS, see TILED_ROWS).
 #ifndef CELLATO_HIP_STREAM_HPP
 #define CELLATO_HIP_STREAM_HPP
 
@@ -30,7 +31,24 @@ constexpr int MAX_PLANES = 5;
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int BLOCK_THREADS = 64 * WAVES_PER_BLOCK;
 
-enum layout_kind : int { LINEAR = 0, TILED = 1 };
+// TILED_ROWS: planes stored in the tiled layout, processed as rows of the linear layout. A lane
+// owns a column of 64 cells, i.e. 8 consecutive 8x8 blocks (8 block words, 64 bytes per block
+// row, moved with four 16-byte accesses). Each lane stages 8 rows per plane in LDS (its own
+// slots, nothing shared) from which the row pipeline reads one row per step; at the first row
+// of a block row the block words loaded earlier are transposed (8x8 bytes, 32 byte permutes)
+// into its 8 rows. Output rows collect in 8 more slots; after the last row of a block row they
+// are transposed back and stored, and the loads of the next block row to stage are issued just
+// before those stores. Loads and stores thus always come in that order, in one branch, and the
+// compiler's wait for the block words counts the stores after them instead of waiting for the
+// stores to complete (with the two bursts in branches of their own, every wait included the
+// stores' completion; one 8-byte access per step instead was 2-4x slower). The stores are
+// plain write-back stores: a lane's 64 bytes go out as four 16-byte pieces of different
+// instructions, which L2 merges into whole lines; written through (device scope, as the other
+// layouts store) each piece is a partial write to HBM, 1.9x slower at depth 2. Everything in between (windows, row sums, shifts) is the linear layout's: a
+// horizontal neighbour costs a funnel shift instead of an exchange with the neighbouring block
+// plus byte-row masks, and the pipeline's halo is T cell rows, not T block rows. Tasks cover
+// whole block rows (chunks of a multiple of 8 rows).
+enum layout_kind : int { LINEAR = 0, TILED = 1, TILED_ROWS = 2 };
 
 // How lanes find the word columns to their left and right.
 enum xch_kind : int {
@@ -390,6 +408,51 @@ constexpr int BUFFER_DWORD3 = 0x00020000;   // raw, untyped 32-bit-element acces
 // it instead of branching around it, so that the compiler's memory wait counts stay exact
 // (a load or store under a branch makes it wait for the whole prefetch queue).
 constexpr uint32_t OOB_OFFSET = 0xFFFFFFF8u;
+// The same for the 16-byte accesses of TILED_ROWS, which add offsets up to 48: those planes are
+// at most OOB_WIDE bytes (checked on the host), so OOB_WIDE + 63 stays past them.
+constexpr uint32_t OOB_WIDE = 0xFFFFFF00u;
+
+using u32x4 = uint32_t __attribute__((ext_vector_type(4)));
+
+// 4x4 byte transpose: o[i] byte k = a[k] byte i (v_perm_b32: bytes 0-3 of the second operand,
+// 4-7 of the first)
+__device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t (&o)[4]) {
+    const uint32_t t0 = __builtin_amdgcn_perm(a1, a0, 0x06020400u);   // a0.0 a1.0 a0.2 a1.2
+    const uint32_t t1 = __builtin_amdgcn_perm(a1, a0, 0x07030501u);   // a0.1 a1.1 a0.3 a1.3
+    const uint32_t t2 = __builtin_amdgcn_perm(a3, a2, 0x06020400u);
+    const uint32_t t3 = __builtin_amdgcn_perm(a3, a2, 0x07030501u);
+    o[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+    o[1] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+    o[2] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+    o[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+// 8x8 byte transpose of 8 words: r[i] byte j = w[j] byte i (its own inverse). For a lane's 8
+// block words (byte i = cell row i of the block) it gives the 8 row words of its 64-cell column.
+__device__ __forceinline__ void transpose8(const word (&w)[8], word (&r)[8]) {
+    uint32_t a[4], b[4], c[4], d[4];
+    transpose4(lo32(w[0]), lo32(w[1]), lo32(w[2]), lo32(w[3]), a);
+    transpose4(lo32(w[4]), lo32(w[5]), lo32(w[6]), lo32(w[7]), b);
+    transpose4(hi32(w[0]), hi32(w[1]), hi32(w[2]), hi32(w[3]), c);
+    transpose4(hi32(w[4]), hi32(w[5]), hi32(w[6]), hi32(w[7]), d);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r[i] = join(a[i], b[i]);
+        r[4 + i] = join(c[i], d[i]);
+    }
+}
+
+// The LDS staging areas of TILED_ROWS: [in / out][wave][plane][row of the block][lane]
+template <int P>
+__device__ __forceinline__ word* rows_stage() {
+    __shared__ word stage[2][WAVES_PER_BLOCK][P][8][64];
+    return &stage[0][0][0][0][0];
+}
+// x + d onto [0, m) for x in [0, m), 0 <= d <= 2m
+__device__ __forceinline__ int wrap_add(int x, int d, int m) {
+    int y = x + d;
+    y -= y >= m ? m : 0;
+    return y - (y >= m ? m : 0);
+}
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const word* p, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<word*>(p), static_cast<short>(0), static_cast<int>(bytes), BUFFER_DWORD3);
@@ -420,6 +483,17 @@ constexpr int prefetch_rows(int T) { return T == 1 ? 6 : T == 2 ? 3 : unroll_row
 // 6 % faster, Critters 2-7 %, WireWorld 1-2 % (T=2 3 % slower), HPP within 2 %. nt (streaming)
 // stores bypass the Infinity Cache that the next launch reads from: 30-50 % slower.
 constexpr int store_aux(int) { return 16; }
+// TILED_ROWS: rows loaded ahead of use (from LDS, so one is enough), chosen per depth to
+// maximise the steps between the loads of a block row (issued with the stores of the block row
+// completed at output step 2T - 1 mod 8) and its transpose into the slots (at the load of its
+// first row, input step T mod 8, made D steps earlier): 1 - T - D mod 8 steps, 8 for 0.
+constexpr int rows_lead(int T, int D) { return ((1 - T - D) % 8 + 16) % 8 == 0 ? 8 : ((1 - T - D) % 8 + 16) % 8; }
+constexpr int rows_prefetch(int T) {
+    int best = 1;
+    for (int d = 1; d <= unroll_rows(T); ++d)
+        if (rows_lead(T, d) > rows_lead(T, best)) best = d;
+    return best;
+}
 
 // Row sums of a linear ring slot: for each counted state K (dictionary index), the mask of
 // the slot's row and its two 1-cell shifts, added as a full adder. Called once per row and
@@ -506,10 +580,13 @@ template <class Eval, int LAYOUT, int T, int X, int HALO, int TPAR = -1, int UR 
 __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     constexpr int P = Eval::planes;
     constexpr bool LOADS_HALO = HALO == HALO_LOAD;
+    constexpr bool ROWS = LAYOUT == TILED_ROWS;     // tiled planes processed as rows
+    constexpr bool LIN = LAYOUT != TILED;           // the linear layout's processing
+    static_assert(!(ROWS && LOADS_HALO), "tiled rows are for the temporal traverser (halo lanes or rings)");
     // lane groups walk different rows: rows narrower than a wave, and the halo-lane tail
     constexpr bool GROUPED = X == XCH_PERMUTE || HALO == HALO_LANES;
     constexpr int U = UR;
-    constexpr int D = PF;
+    constexpr int D = ROWS ? rows_prefetch(T) : PF;
     static_assert(UR % 3 == 0 && PF >= 1 && PF <= UR, "rows per body: a multiple of 3; prefetch at most one body");
     static_assert(!(HALO == HALO_LOAD && T != 1), "loaded halos are exact for one generation only");
 
@@ -563,7 +640,9 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
         rin[p] = plane_rsrc(a.in[p], plane_bytes);
         rout[p] = plane_rsrc(a.out[p], plane_bytes);
     }
-    const uint32_t col_off = static_cast<uint32_t>(col) * 8u;
+    // a linear row is wx words; in TILED_ROWS a lane's column is a span of 8 block words (64
+    // bytes) and a block row holds 8 cell rows (8 x row_bytes)
+    const uint32_t col_off = static_cast<uint32_t>(col) * (ROWS ? 64u : 8u);
     // the end lanes' outside column; the other lanes read past the plane. The row offset is
     // added with saturation and capped at OOB_OFFSET: no per-row select, which the compiler
     // had turned into a branch around the load in every row.
@@ -620,8 +699,71 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     // go to the offset past the plane (they read 0 and move no data); no step uses them.
     int lrow_next = wrap(r0 - T, a.h);
     int lstep = 0;
+    // TILED_ROWS: the block words of the next block row to stage, loaded ahead
+    uint32_t blk[ROWS ? P : 1][16];
+    word* stage_in = nullptr;    // this wave's slots: input rows, output rows
+    word* stage_out = nullptr;
+    // block row starting at row `first` (a multiple of 8) into blk, or nothing
+    auto load_block = [&](int first, bool want) {
+        const uint32_t off = want ? static_cast<uint32_t>(first >> 3) * (row_bytes * 8u) + col_off : OOB_WIDE;
+#pragma unroll
+        for (int p = 0; p < (ROWS ? P : 0); ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rin[p], static_cast<int>(off), 16 * q, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) blk[p][4 * q + e] = v[e];
+            }
+    };
+    auto stage_block = [&]() {   // blk -> its 8 rows, into the input slots
+#pragma unroll
+        for (int p = 0; p < (ROWS ? P : 0); ++p) {
+            word w[8], r[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = join(blk[p][2 * j], blk[p][2 * j + 1]);
+            transpose8(w, r);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) stage_in[(p * 8 + i) * 64 + lane] = r[i];
+        }
+    };
+    // Call c of load_row reads row (c - T) & 7 of its block row (r0 and h are multiples of 8:
+    // the same in every lane group); the call reading row 0 (c = 8 - K0 + 8m, made at step
+    // c - D) first stages the block row. The loads for a staging are issued at the end of the
+    // last output step 2T - 1 + 8m before it, rows_lead(T, D) steps earlier.
+    constexpr int K0 = (8 - T % 8) % 8;              // row of call 0 in its block row
+    constexpr int S1 = 8 - K0 - D;                   // step of the first staging after call 0's
+    constexpr int C0 = (2 * T + 7) % 8;              // first output step that stores a block row
+    constexpr int LEAD = rows_lead(T, D);
+    if constexpr (ROWS) {
+        stage_in = rows_stage<P>() + (static_cast<int>(threadIdx.x) / 64) * (P * 512);
+        stage_out = stage_in + WAVES_PER_BLOCK * P * 512;
+        const int first = lrow_next - K0;
+        load_block(first, true);
+        stage_block();
+        // the block row of the first staging, when it comes before the first output block
+        if constexpr (S1 <= C0) load_block(wrap_add(first, 8, a.h), 8 - K0 < nsteps);
+        // stores after the loads, as in the loop, so that the wait for blk is the same on
+        // every path into it (these go past the plane)
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                __builtin_amdgcn_raw_buffer_store_b128(u32x4{0u, 0u, 0u, 0u}, rout[p], static_cast<int>(OOB_WIDE), 16 * q, 0);
+    }
     auto load_row = [&](word (&dst)[P], word (&hdst)[P]) {
         const bool live = lstep < nsteps;   // uniform across the wave
+        if constexpr (ROWS) {
+            const int k = (lstep - T) & 7;
+            if (lstep > 0 && k == 0) stage_block();
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                dst[p] = stage_in[(p * 8 + k) * 64 + lane];
+                hdst[p] = 0;
+            }
+            lrow_next = lrow_next + 1 == a.h ? 0 : lrow_next + 1;
+            ++lstep;
+            return;
+        }
         const uint32_t roff = static_cast<uint32_t>(lrow_next) * row_bytes;
         const uint32_t coff = live ? col_off : OOB_OFFSET;
 #pragma unroll
@@ -657,7 +799,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             ycur = ycur + 1 == a.h ? 0 : ycur + 1;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
-                if constexpr (LAYOUT == LINEAR) {
+                if constexpr (LIN) {
                     const word v = pf[u][p];
                     lwin[0][nw].w[p] = v;
                     if constexpr (LOADS_HALO) {
@@ -686,7 +828,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     }
                 }
             }
-            if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
+            if constexpr (LIN && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
             if constexpr (LAYOUT == TILED && NS > 0)
                 tiled_sum_fill<targets>::template run<EDGES, LOADS_HALO>(win[0][nw], wedge[0][nw], xc, tsum[0][nw]);
             // the load stays after this step's use of the window (hoisted above it, it would
@@ -703,7 +845,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             for (int k = 0; k < T; ++k) {
                 if (FILL && ib + u < 2 * k + 2) continue;
                 const int t = TPAR < 0 ? a.t0 + k : (TPAR + k) & 1;
-                if constexpr (LAYOUT == LINEAR) {
+                if constexpr (LIN) {
                     // the row of stage k (only rules with a Margolus block read it): one
                     // conditional add, k mod h being a loop-invariant scalar (a loop here made
                     // every stage branch)
@@ -723,7 +865,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     const int kn = k + 1 < T ? k + 1 : 0;
 #pragma unroll
                     for (int p = 0; p < P; ++p) {
-                        if constexpr (LAYOUT == LINEAR) {
+                        if constexpr (LIN) {
                             lwin[kn][nw].w[p] = out.p[p];
                             lwin[kn][nw].l[p] = xc.left0(hi32(out.p[p]));
                             lwin[kn][nw].r[p] = xc.right0(lo32(out.p[p]));
@@ -735,7 +877,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                                                                join(xc.right0(lo32(v)), xc.right0(hi32(v))));
                         }
                     }
-                    if constexpr (LAYOUT == LINEAR && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
+                    if constexpr (LIN && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
                     if constexpr (LAYOUT == TILED && NS > 0)
                         tiled_sum_fill<targets>::template run<EDGES, false>(win[kn][nw], wedge[kn][nw], xc, tsum[kn][nw]);
                 }
@@ -743,9 +885,34 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             // generation T of row s - T (in [r0, r1) implies in [0, h))
             const int yo = s - T;
             const bool stored = (yo >= r0) & (yo < r1) & owner;   // no short circuit (see OOB_OFFSET)
-            const uint32_t soff = stored ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
+            if constexpr (ROWS) {
+                // output row o of its block row into slot o; after row 7 the loads of the next
+                // block row to stage (the one whose row 0 is read at step + LEAD + D, i.e. row
+                // yo + T + LEAD + D), then the block row's 8 words out
+                const int o = (ib + u - 2 * T) & 7;
 #pragma unroll
-            for (int p = 0; p < P; ++p) buf_store<SAUX>(out.p[p], rout[p], soff, 0u);
+                for (int p = 0; p < P; ++p) stage_out[(p * 8 + o) * 64 + lane] = out.p[p];
+                if (o == 7) {
+                    load_block(wrap(yo + T + LEAD + D, a.h), ib + u + LEAD + D < nsteps);
+                    const uint32_t boff = stored ? static_cast<uint32_t>(yo >> 3) * (row_bytes * 8u) + col_off : OOB_WIDE;
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        word r[8], w[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) r[i] = stage_out[(p * 8 + i) * 64 + lane];
+                        transpose8(r, w);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const u32x4 v = {lo32(w[2 * q]), hi32(w[2 * q]), lo32(w[2 * q + 1]), hi32(w[2 * q + 1])};
+                            __builtin_amdgcn_raw_buffer_store_b128(v, rout[p], static_cast<int>(boff), 16 * q, 0);
+                        }
+                    }
+                }
+            } else {
+                const uint32_t soff = stored ? static_cast<uint32_t>(yo) * row_bytes + col_off : OOB_OFFSET;
+#pragma unroll
+                for (int p = 0; p < P; ++p) buf_store<SAUX>(out.p[p], rout[p], soff, 0u);
+            }
         }
     };
     // The fill runs in a copy of the loop that tests each stage, the steady state in one

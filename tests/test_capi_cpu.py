@@ -93,6 +93,28 @@ def test_plan_fills_every_simd_in_whole_unrolled_blocks(T):
     assert _lib.plan(0, 0, 16384, 16384, T, 7)[0] == 7
 
 
+def test_plan_tiled_temporal_passes_as_rows_or_blocks():
+    # Game of Life at depth 4 takes the tiled-as-rows kernels: 16384 cells are 256 columns of
+    # 64 cells, four 62-column halo-lane segments and 8 columns in groups of 10 lanes (6 per
+    # wave), each task a whole number of block rows (chunks reported in block rows)
+    import math
+    chunk, tasks, blocks = _lib.plan(0, 1, 16384, 16384, 4)
+    assert tasks == 4 * math.ceil(16384 / (8 * chunk)) + math.ceil(16384 / (6 * 8 * chunk))
+    assert blocks == math.ceil(tasks / 4)
+    assert _lib.plan(0, 1, 16384, 16384, 4, 5)[0] == 5
+    # below its crossover depth, for WireWorld (no such kernels) and for rows of fewer than 64
+    # columns of 64 cells: 8x8 blocks, 2048 block columns = 33 segments + 2 columns in groups
+    # of 4 lanes (16 per wave)
+    for rule, T, width in [(0, 2, 16384), (1, 4, 16384), (0, 4, 2048)]:
+        chunk, tasks, _ = _lib.plan(rule, 1, width, 16384, T)
+        wx = width // 8
+        nseg, left = wx // 62, wx % 62
+        want = nseg * math.ceil(2048 / chunk)
+        if left:
+            want += math.ceil(2048 / ((64 // (left + 2)) * chunk))
+        assert tasks == want, (rule, T, width)
+
+
 def test_plan_rejects_what_the_traversers_reject():
     for args, code in [((11, 0, 128, 16, 0), _lib.E_RULE), ((0, 3, 128, 16, 0), _lib.E_LAYOUT),
                        ((0, 0, 100, 16, 0), _lib.E_GEOMETRY), ((0, 0, 128, 16, 9), _lib.E_TEMPORAL)]:

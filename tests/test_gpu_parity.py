@@ -16,10 +16,12 @@ pytestmark = pytest.mark.gpu
 
 RULES = ["game-of-life", "wire", "forest-fire", "excitable", "brian", "maze", "traffic", "fluid", "critters", "cyclic"]
 LINEAR_SHAPES = [(192, 70), (4096, 40), (8192, 50), (8320, 33), (4160, 20), (16384, 24), (7936, 12), (128, 1), (64, 3)]
-# 4096: one ring of 64 block columns x 8; 4160, 8320, 4104: segments of 62 blocks and a tail of
-# narrow lane groups
+# 4104: segments of 62 block columns and a tail of narrow lane groups. Widths of at least 64
+# columns of 64 cells run the temporal traverser of rules with at most two planes as rows:
+# 4096 one ring of 64 columns (also 8 rows high, a single block row), 4160, 8320 and 12352
+# segments of 62 columns and a tail (3, 6 and 7 columns)
 TILED_SHAPES = [(40, 24), (512, 16), (1024, 24), (1040, 32), (520, 40), (8, 8), (4096, 40), (4160, 16), (8320, 24),
-                (4104, 16)]
+                (4104, 16), (4096, 8), (12352, 56)]
 TRAVERSERS = [("simple", 1), ("temporal", 1), ("temporal", 2), ("temporal", 3), ("temporal", 5), ("temporal", 8)]
 
 
