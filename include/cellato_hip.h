@@ -58,14 +58,17 @@ int cellato_max_temporal_steps(void);
  * are ping-ponged; on return *result_in_b tells which one holds the result (buf_a's
  * contents are overwritten whenever steps >= 2). Launches are asynchronous on `stream`
  * (a hipStream_t, NULL = default stream).
- * chunk_rows: rows of words a wave streams through per task; 0 picks a default.
+ * chunk_rows: rows of words a wave streams through per task (tiled: rows of 8x8 blocks);
+ * 0 picks a default.
  *
  * simple:   one generation per kernel launch.
  * temporal: temporal_steps generations per kernel launch (register-resident pipeline);
  *           steps need not be a multiple of temporal_steps (the remainder runs as a
  *           shorter last launch). A depth whose kernel would need scratch memory (the
  *           cyclic automaton's linear layout above 4) runs as launches of the deepest depth
- *           that does not; cellato_hip_kernel_info reports the depth used.
+ *           that does not; cellato_hip_kernel_info reports the depth used. Over the tiled
+ *           layout, rules of one or two planes from a depth measured per automaton run the
+ *           planes as rows of 64-cell columns (same results, same arguments).
  * Geometry: linear needs width % 64 == 0; tiled needs width % 8 == 0 and height % 8 == 0. */
 int cellato_hip_simple(int rule, int layout, uint64_t* const* buf_a, uint64_t* const* buf_b,
                        int width, int height, int t0, int steps, int chunk_rows,
