@@ -497,28 +497,38 @@ constexpr int rows_prefetch(int T) {
 
 // Row sums of a linear ring slot: for each counted state K (dictionary index), the mask of
 // the slot's row and its two 1-cell shifts, added as a full adder. Called once per row and
-// stage, when the row enters the ring.
+// stage, when the row enters the ring. The neighbouring columns' edge bits of the mask come
+// from the neighbouring lanes' masks (one lane exchange per direction and counted state,
+// instead of one per plane followed by the mask of the exchanged halves); end lanes that load
+// the outside column (HALO) take its mask instead.
 template <class Targets>
 struct row_sum_fill;
 template <long long... K>
 struct row_sum_fill<klist<K...>> {
-    template <int P, int NS>
-    __device__ __forceinline__ static void run(lrow<P, NS>& r) {
-        (one<K, kindex<K, klist<K...>>::value>(r), ...);
+    template <bool HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void run(lrow<P, NS>& r, const XC& xc, const word (&halo)[P]) {
+        (one<K, kindex<K, klist<K...>>::value, HALO>(r, xc, halo), ...);
     }
-    template <long long Kj, int j, int P, int NS>
-    __device__ __forceinline__ static void one(lrow<P, NS>& r) {
+    template <long long Kj, int j, bool HALO, int P, int NS, class XC>
+    __device__ __forceinline__ static void one(lrow<P, NS>& r, const XC& xc, const word (&halo)[P]) {
         using namespace cellato::evaluators::bit_planes;
         const is_index<Kj> f{};
-        vint<P> own, side_l, side_r;
+        vint<P> own;
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            own.p[p] = r.w[p];
-            side_l.p[p] = join(0u, r.l[p]);
-            side_r.p[p] = join(r.r[p], 0u);
-        }
+        for (int p = 0; p < P; ++p) own.p[p] = r.w[p];
         const word m = f(own);
-        const uint32_t lh = hi32(f(side_l)), rl = lo32(f(side_r));
+        uint32_t lh, rl;
+        if constexpr (HALO) {
+            vint<P> hv;
+#pragma unroll
+            for (int p = 0; p < P; ++p) hv.p[p] = halo[p];
+            const word mh = f(hv);
+            lh = xc.left(hi32(m), hi32(mh));
+            rl = xc.right(lo32(m), lo32(mh));
+        } else {
+            lh = xc.left0(hi32(m));
+            rl = xc.right0(lo32(m));
+        }
         fa(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl), r.rs_s[j], r.rs_c[j]);
     }
 };
@@ -828,7 +838,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     }
                 }
             }
-            if constexpr (LIN && NS > 0) row_sum_fill<targets>::run(lwin[0][nw]);
+            if constexpr (LIN && NS > 0) row_sum_fill<targets>::template run<LOADS_HALO>(lwin[0][nw], xc, hpf[u]);
             if constexpr (LAYOUT == TILED && NS > 0)
                 tiled_sum_fill<targets>::template run<EDGES, LOADS_HALO>(win[0][nw], wedge[0][nw], xc, tsum[0][nw]);
             // the load stays after this step's use of the window (hoisted above it, it would
@@ -877,7 +887,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                                                                join(xc.right0(lo32(v)), xc.right0(hi32(v))));
                         }
                     }
-                    if constexpr (LIN && NS > 0) row_sum_fill<targets>::run(lwin[kn][nw]);
+                    if constexpr (LIN && NS > 0) row_sum_fill<targets>::template run<false>(lwin[kn][nw], xc, hpf[u]);
                     if constexpr (LAYOUT == TILED && NS > 0)
                         tiled_sum_fill<targets>::template run<EDGES, false>(win[kn][nw], wedge[kn][nw], xc, tsum[kn][nw]);
                 }
