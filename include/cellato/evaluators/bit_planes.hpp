@@ -926,6 +926,12 @@ template <class H, class... T, class X>
 struct tl_index<tlist<H, T...>, X> {
     static constexpr int value = std::is_same_v<H, X> ? 0 : (tl_index<tlist<T...>, X>::value < 0 ? -1 : 1 + tl_index<tlist<T...>, X>::value);
 };
+template <class L, int I>
+struct tl_at;
+template <class H, class... T>
+struct tl_at<tlist<H, T...>, 0> { using type = H; };
+template <class H, class... T, int I>
+struct tl_at<tlist<H, T...>, I> { using type = typename tl_at<tlist<T...>, I - 1>::type; };
 
 // pt_pred: a boolean node that reads the neighbourhood and not the cell
 template <class Dict, class E>
@@ -1037,17 +1043,17 @@ struct pt_cell_tests<N<A...>> { static constexpr int value = (0 + ... + pt_cell_
 template <auto V>
 struct pt_cell_tests<equals<neighbor_at<0, 0>, state_constant<V>>> { static constexpr int value = 1; };
 
-// Taken for rules that compare the cell with three states or more: shorter chains the compiler
-// already folds into the cell tests (measured VALU per launch at 16384^2, table / node by node:
-// Greenberg-Hastings 0.55-0.57, forest fire 0.68-0.69, WireWorld 0.95-0.96, Brian's Brain with
-// two comparisons 1.12).
+// Taken for rules that compare the cell with two states or more (measured VALU per launch at
+// 16384^2, table / node by node: Greenberg-Hastings 0.55-0.57, forest fire 0.68-0.69,
+// WireWorld 0.95-0.96; Brian's Brain, with two comparisons, 1.12 until the table's counts could
+// include the cell (self_free below), 0.97 since: loop VALU per word and stage 30.5 -> 29.7).
 template <class Dict, class E, int P>
 struct pt_rule {
     using X = pt_expr<Dict, E, tlist<>>;
     using preds = typename X::preds;
     static constexpr int M = preds::size;
     static constexpr int n = P + M;   // inputs: the cell's planes, then the predicates
-    static constexpr bool ok = X::ok && M >= 1 && n <= 6 && pt_cell_tests<E>::value >= 3;
+    static constexpr bool ok = X::ok && M >= 1 && n <= 6 && pt_cell_tests<E>::value >= 2;
     // truth table of result plane `b` (bit i: inputs i, cell index in the low P bits) and the
     // inputs that do not occur (cell indices past the dictionary)
     static constexpr unsigned long long table(int b) {
@@ -1065,7 +1071,52 @@ struct pt_rule {
             if ((i & ((1 << P) - 1)) >= Dict::number_of_values) f |= 1ull << i;
         return f;
     }
+    // whether predicate i changes the result at a cell of index c (for some value of the others)
+    static constexpr bool pred_read_at(int i, long long c) {
+        for (unsigned rest = 0; rest < (1u << M); ++rest)
+            if (!((rest >> i) & 1u) && pt_at<Dict, E, preds>::at(c, rest) != pt_at<Dict, E, preds>::at(c, rest | (1u << i)))
+                return true;
+        return false;
+    }
+    // predicate i may count the cell itself: it is read only at cells that are none of the
+    // states it counts
+    template <int i, long long... K>
+    static constexpr bool self_free(klist<K...>) { return !(pred_read_at(i, K) || ...); }
 };
+
+// the states a predicate counts (dictionary indices)
+template <class Dict, class E>
+struct pt_counted { using type = klist<>; };
+template <class Dict, template <class...> class N, class... A>
+struct pt_counted<Dict, N<A...>> { using type = typename kjoin_all<typename pt_counted<Dict, A>::type...>::type; };
+template <class Dict, auto V, class N>
+struct pt_counted<Dict, count_neighbors<state_constant<V>, N>> { using type = klist<index_of<Dict, V>>; };
+
+// A window whose own cell matches nothing: a Moore count read through it includes the cell
+// (the own row's 3-cell sum is not reduced by the cell). Exact for a predicate that the rule
+// reads only at cells that are not the counted state, and two ops per 32 bits cheaper.
+template <class Win>
+struct self_incl_window {
+    const Win& w;
+    int t;
+    static constexpr bool row_sums = has_row_sums<Win>();
+    CELLATO_HD word yodd() const { return w.yodd(); }
+    template <long long K, int dy>
+    CELLATO_HD void rsum(word& s, word& c) const { w.template rsum<K, dy>(s, c); }
+    template <int dx, int dy>
+    CELLATO_HD auto nb() const { return w.template nb<dx, dy>(); }
+    template <int dx, int dy, class F>
+    CELLATO_HD word nbf(F f) const {
+        if constexpr (dx == 0 && dy == 0) return word(0);
+        else return w.template nbf<dx, dy>(f);
+    }
+};
+
+template <bool SelfFree, class Dict, class Pr, class Win>
+CELLATO_HD word pt_pred_value(const Win& w) {
+    if constexpr (SelfFree) return ev<Dict, Pr>::at(self_incl_window<Win>{w, w.t});
+    else return ev<Dict, Pr>::at(w);
+}
 
 // A boolean function of the first N inputs in[0..N-1] (truth table F, free entries D) from
 // three-input ops: one op for N <= 3, else a select on the last input between the two halves.
@@ -1111,8 +1162,9 @@ CELLATO_HD vint<P> pt_eval(const Win& w, tlist<Pr...>) {
     const vint<P> cell = w.template nb<0, 0>();
 #pragma unroll
     for (int i = 0; i < P; ++i) in[i] = cell.p[i];
-    int k = P;
-    ((in[k++] = ev<Dict, Pr>::at(w)), ...);
+    [&]<int... I>(std::integer_sequence<int, I...>) {
+        ((in[P + I] = pt_pred_value<R::template self_free<I>(typename pt_counted<Dict, Pr>::type{}), Dict, Pr>(w)), ...);
+    }(std::make_integer_sequence<int, sizeof...(Pr)>{});
     vint<P> r;
     [&]<int... B>(std::integer_sequence<int, B...>) {
         ((r.p[B] = pt_fn<R::n, R::table(B), R::free_inputs()>(in)), ...);

@@ -24,7 +24,7 @@ static_assert(table_path<cellato::automata::GAME_OF_LIFE> && table_path<cellato:
 static_assert(!table_path<cellato::automata::WIRE> && !table_path<cellato::automata::CRITTERS> &&
               !table_path<cellato::automata::BRIAN> && !table_path<cellato::automata::CYCLIC>);
 // Rules of several planes whose neighbourhood enters only through predicates of neighbour
-// counts, and that compare the cell with three states or more, take the predicate-table path
+// counts, and that compare the cell with two states or more, take the predicate-table path
 // (every window); the others do not.
 template <int id>
 constexpr bool predicate_path = cellato::automata::by_id<id>::config::state_dictionary::needed_bits > 1 &&
@@ -32,10 +32,22 @@ constexpr bool predicate_path = cellato::automata::by_id<id>::config::state_dict
                 typename cellato::automata::by_id<id>::config::algorithm,
                 cellato::automata::by_id<id>::config::state_dictionary::needed_bits>::ok;
 static_assert(predicate_path<cellato::automata::WIRE> && predicate_path<cellato::automata::EXCITABLE> &&
-              predicate_path<cellato::automata::FOREST_FIRE>);
+              predicate_path<cellato::automata::FOREST_FIRE> && predicate_path<cellato::automata::BRIAN>);
 static_assert(!predicate_path<cellato::automata::TRAFFIC> && !predicate_path<cellato::automata::FLUID> &&
-              !predicate_path<cellato::automata::CYCLIC> && !predicate_path<cellato::automata::GAME_OF_LIFE> &&
-              !predicate_path<cellato::automata::BRIAN>);   // two cell comparisons: node by node
+              !predicate_path<cellato::automata::CYCLIC> && !predicate_path<cellato::automata::GAME_OF_LIFE>);
+// WireWorld reads its head count only at conductors, Greenberg-Hastings its excited count only
+// at resting cells and Brian's Brain its count of firing cells only at off cells: their
+// predicates count through windows that include the cell itself
+template <int id, int i>
+constexpr bool self_free_pred = [] {
+    using C = typename cellato::automata::by_id<id>::config;
+    using D = typename C::state_dictionary;
+    using R = bp::pt_rule<D, typename C::algorithm, D::needed_bits>;
+    using Pr = typename bp::tl_at<typename R::preds, i>::type;
+    return R::template self_free<i>(typename bp::pt_counted<D, Pr>::type{});
+}();
+static_assert(self_free_pred<cellato::automata::WIRE, 0> && self_free_pred<cellato::automata::EXCITABLE, 0> &&
+              self_free_pred<cellato::automata::BRIAN, 0>);
 
 using namespace cellato::evaluators::bit_planes;
 using u64 = std::uint64_t;
