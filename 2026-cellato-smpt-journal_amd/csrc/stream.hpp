@@ -797,6 +797,12 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
     int kmod[T];
 #pragma unroll
     for (int k = 0; k < T; ++k) kmod[k] = k % a.h;
+    // parity of stage k's row y = ycur - k mod h: that of ycur, flipped for odd k mod h, and
+    // flipped again where the subtraction wraps (ycur < k mod h) on a torus of odd height
+    const bool hodd = (a.h & 1) != 0;
+    uint32_t kflip[T];
+#pragma unroll
+    for (int k = 0; k < T; ++k) kflip[k] = (kmod[k] & 1) ? ~0u : 0u;
 
     // One block of U steps. FILL: the block may fall in the pipeline fill (steps below 2T),
     // where stages that no stored row needs are skipped.
@@ -807,6 +813,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
             const int s = r0 - T + ib + u;
             const int nw = u % 3, tp = (u + 1) % 3, md = (u + 2) % 3;   // slots: newest, y-1, y
             ycur = ycur + 1 == a.h ? 0 : ycur + 1;
+            const uint32_t ypar = (ycur & 1) ? ~0u : 0u;
 #pragma unroll
             for (int p = 0; p < P; ++p) {
                 if constexpr (LIN) {
@@ -856,12 +863,11 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                 if (FILL && ib + u < 2 * k + 2) continue;
                 const int t = TPAR < 0 ? a.t0 + k : (TPAR + k) & 1;
                 if constexpr (LIN) {
-                    // the row of stage k (only rules with a Margolus block read it): one
-                    // conditional add, k mod h being a loop-invariant scalar (a loop here made
-                    // every stage branch)
-                    int y = ycur - kmod[k];
-                    y += y < 0 ? a.h : 0;
-                    const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], (y & 1) ? ~word(0) : word(0), t};
+                    // the parity of the row of stage k (only rules with a Margolus block read
+                    // it): one compare and one select per stage (forming the row, y - k wrapped,
+                    // took five more ops per word and stage)
+                    const uint32_t yk = ypar ^ kflip[k] ^ ((hodd & (ycur < kmod[k])) ? ~0u : 0u);
+                    const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], join(yk, yk), t};
                     out = Eval::evaluate(wnd);
                 } else {
                     // without edge words, the halo column (HALO_LOAD) is kept in stage 0's slots only
