@@ -500,11 +500,13 @@ constexpr int rows_prefetch(int T) {
 // stage, when the row enters the ring. The neighbouring columns' edge bits of the mask come
 // from the neighbouring lanes' masks (one lane exchange per direction and counted state,
 // instead of one per plane followed by the mask of the exchanged halves); end lanes that load
-// the outside column (HALO) take its mask instead.
-template <class Targets>
+// the outside column (HALO) take its mask instead. A state the rule reads only as "some
+// neighbour is K" (Eval::row_sum_or_only) keeps the 3-cell OR in the sum word and 0 in the
+// carry word: one op instead of two, and a carry word that is a constant, not a register.
+template <class Eval, class Targets>
 struct row_sum_fill;
-template <long long... K>
-struct row_sum_fill<klist<K...>> {
+template <class Eval, long long... K>
+struct row_sum_fill<Eval, klist<K...>> {
     template <bool HALO, int P, int NS, class XC>
     __device__ __forceinline__ static void run(lrow<P, NS>& r, const XC& xc, const word (&halo)[P]) {
         (one<K, kindex<K, klist<K...>>::value, HALO>(r, xc, halo), ...);
@@ -529,7 +531,12 @@ struct row_sum_fill<klist<K...>> {
             lh = xc.left0(hi32(m));
             rl = xc.right0(lo32(m));
         }
-        fa(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl), r.rs_s[j], r.rs_c[j]);
+        if constexpr (Eval::template row_sum_or_only<Kj>) {
+            r.rs_s[j] = lop3<0xFE>(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl));
+            r.rs_c[j] = 0;
+        } else {
+            fa(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl), r.rs_s[j], r.rs_c[j]);
+        }
     }
 };
 
@@ -845,7 +852,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     }
                 }
             }
-            if constexpr (LIN && NS > 0) row_sum_fill<targets>::template run<LOADS_HALO>(lwin[0][nw], xc, hpf[u]);
+            if constexpr (LIN && NS > 0) row_sum_fill<Eval, targets>::template run<LOADS_HALO>(lwin[0][nw], xc, hpf[u]);
             if constexpr (LAYOUT == TILED && NS > 0)
                 tiled_sum_fill<targets>::template run<EDGES, LOADS_HALO>(win[0][nw], wedge[0][nw], xc, tsum[0][nw]);
             // the load stays after this step's use of the window (hoisted above it, it would
@@ -893,7 +900,7 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                                                                join(xc.right0(lo32(v)), xc.right0(hi32(v))));
                         }
                     }
-                    if constexpr (LIN && NS > 0) row_sum_fill<targets>::template run<false>(lwin[kn][nw], xc, hpf[u]);
+                    if constexpr (LIN && NS > 0) row_sum_fill<Eval, targets>::template run<false>(lwin[kn][nw], xc, hpf[u]);
                     if constexpr (LAYOUT == TILED && NS > 0)
                         tiled_sum_fill<targets>::template run<EDGES, false>(win[kn][nw], wedge[kn][nw], xc, tsum[kn][nw]);
                 }

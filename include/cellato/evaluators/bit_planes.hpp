@@ -1082,6 +1082,9 @@ struct pt_rule {
     // states it counts
     template <int i, long long... K>
     static constexpr bool self_free(klist<K...>) { return !(pred_read_at(i, K) || ...); }
+    // every predicate that counts state K may count the cell itself
+    template <long long K>
+    static constexpr bool self_free_counts();
 };
 
 // the states a predicate counts (dictionary indices)
@@ -1091,6 +1094,27 @@ template <class Dict, template <class...> class N, class... A>
 struct pt_counted<Dict, N<A...>> { using type = typename kjoin_all<typename pt_counted<Dict, A>::type...>::type; };
 template <class Dict, auto V, class N>
 struct pt_counted<Dict, count_neighbors<state_constant<V>, N>> { using type = klist<index_of<Dict, V>>; };
+
+template <class Dict, class E, int P>
+template <long long K>
+constexpr bool pt_rule<Dict, E, P>::self_free_counts() {
+    return []<int... I>(std::integer_sequence<int, I...>) {
+        return ((kindex<K, typename pt_counted<Dict, typename tl_at<preds, I>::type>::type>::value < 0 ||
+                 self_free<I>(typename pt_counted<Dict, typename tl_at<preds, I>::type>::type{})) && ...);
+    }(std::make_integer_sequence<int, M>{});
+}
+
+// whether every Moore count of state K in E is the test "count > 0" (any neighbour)
+template <class Dict, class E, long long K>
+struct any_only : std::true_type {};
+template <class Dict, template <class...> class N, class... A, long long K>
+struct any_only<Dict, N<A...>, K> : std::bool_constant<(any_only<Dict, A, K>::value && ...)> {};
+template <class Dict, auto V, long long K>
+struct any_only<Dict, count_neighbors<state_constant<V>, moore_8_neighbors>, K>
+    : std::bool_constant<index_of<Dict, V> != K> {};
+template <class Dict, auto V, auto Z, long long K>
+struct any_only<Dict, greater_than<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<Z>>, K>
+    : std::bool_constant<Z == 0 || index_of<Dict, V> != K> {};
 
 // A window whose own cell matches nothing: a Moore count read through it includes the cell
 // (the own row's 3-cell sum is not reduced by the cell). Exact for a predicate that the rule
@@ -1185,6 +1209,17 @@ struct evaluator {
     static constexpr int layout = 0;
     // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of
     using row_sum_targets = typename row_targets<state_dictionary_type, Expression>::type;
+    // a counted state read only as "some neighbour is K", through the predicate table and
+    // counting the cell itself: its row sum may be kept as the 3-cell OR alone (the sum word
+    // holding the OR, the carry word 0), one word per ring slot instead of two
+    template <long long K>
+    static constexpr bool row_sum_or_only = [] {
+        if constexpr (planes > 1 && pt_rule<state_dictionary_type, Expression, planes>::ok)
+            return any_only<state_dictionary_type, Expression, K>::value &&
+                   pt_rule<state_dictionary_type, Expression, planes>::template self_free_counts<K>();
+        else
+            return false;
+    }();
     // the rule reads the parity of the time step (alternating algorithms, Margolus blocks):
     // a streaming caller may then pass a window whose `t` is only that parity
     static constexpr bool reads_step_parity =

@@ -48,6 +48,13 @@ constexpr bool self_free_pred = [] {
 }();
 static_assert(self_free_pred<cellato::automata::WIRE, 0> && self_free_pred<cellato::automata::EXCITABLE, 0> &&
               self_free_pred<cellato::automata::BRIAN, 0>);
+// Greenberg-Hastings reads its excited count only as "some neighbour is excited": its row sums
+// may be kept as 3-cell ORs; WireWorld and Brian's Brain compare their counts with 1, 2
+template <int id, long long K>
+constexpr bool or_sums = bp::evaluator<std::uint64_t, typename cellato::automata::by_id<id>::config::state_dictionary,
+                                       typename cellato::automata::by_id<id>::config::algorithm>::template row_sum_or_only<K>;
+static_assert(or_sums<cellato::automata::EXCITABLE, 1> && !or_sums<cellato::automata::WIRE, 1> &&
+              !or_sums<cellato::automata::BRIAN, 2> && !or_sums<cellato::automata::GAME_OF_LIFE, 1>);
 
 using namespace cellato::evaluators::bit_planes;
 using u64 = std::uint64_t;
