@@ -1136,10 +1136,75 @@ struct self_incl_window {
     }
 };
 
+template <int N, unsigned long long F, unsigned long long D>
+CELLATO_HD word pt_fn(const word* in);
+
+// A compound predicate (and / or / not) of comparisons of one state's Moore count with
+// constants: its value as a function of the count, cp_eval<..>::at(n).
+template <class Dict, class E>
+struct cp_eval { static constexpr bool ok = false; };
+#define CELLATO_CP_CMP(node, op)                                                                \
+    template <class Dict, auto V, auto C>                                                       \
+    struct cp_eval<Dict, node<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<C>>> { \
+        static constexpr bool ok = true;                                                        \
+        static constexpr bool at(long long n) { return n op static_cast<long long>(C); }        \
+    };
+CELLATO_CP_CMP(equals, ==)
+CELLATO_CP_CMP(not_equals, !=)
+CELLATO_CP_CMP(greater_than, >)
+CELLATO_CP_CMP(less_than, <)
+#undef CELLATO_CP_CMP
+template <class Dict, class L, class R>
+struct cp_eval<Dict, and_<L, R>> {
+    static constexpr bool ok = cp_eval<Dict, L>::ok && cp_eval<Dict, R>::ok;
+    static constexpr bool at(long long n) { return cp_eval<Dict, L>::at(n) && cp_eval<Dict, R>::at(n); }
+};
+template <class Dict, class L, class R>
+struct cp_eval<Dict, or_<L, R>> {
+    static constexpr bool ok = cp_eval<Dict, L>::ok && cp_eval<Dict, R>::ok;
+    static constexpr bool at(long long n) { return cp_eval<Dict, L>::at(n) || cp_eval<Dict, R>::at(n); }
+};
+template <class Dict, class V>
+struct cp_eval<Dict, not_<V>> {
+    static constexpr bool ok = cp_eval<Dict, V>::ok;
+    static constexpr bool at(long long n) { return !cp_eval<Dict, V>::at(n); }
+};
+template <class E> struct cp_compound : std::false_type {};
+template <class L, class R> struct cp_compound<and_<L, R>> : std::true_type {};
+template <class L, class R> struct cp_compound<or_<L, R>> : std::true_type {};
+template <class V> struct cp_compound<not_<V>> : std::true_type {};
+template <class L> struct kfirst;
+template <long long H, long long... T> struct kfirst<klist<H, T...>> { static constexpr long long value = H; };
+
+// Truth table of such a predicate over the adder form of the count, inputs (b0, s1, k, m1)
+// (index b0 + 2 s1 + 4 k + 8 m1; count = b0 + 2 (s1 + k + 2 m1))
+template <class Dict, class Pr>
+constexpr unsigned long long cp_table() {
+    unsigned long long t = 0;
+    for (int i = 0; i < 16; ++i) {
+        const long long n = (i & 1) + 2 * (((i >> 1) & 1) + ((i >> 2) & 1) + 2 * ((i >> 3) & 1));
+        if (cp_eval<Dict, Pr>::at(n)) t |= 1ull << i;
+    }
+    return t;
+}
+
 template <bool SelfFree, class Dict, class Pr, class Win>
 CELLATO_HD word pt_pred_value(const Win& w) {
-    if constexpr (SelfFree) return ev<Dict, Pr>::at(self_incl_window<Win>{w, w.t});
-    else return ev<Dict, Pr>::at(w);
+    if constexpr (has_row_sums<Win>() && cp_compound<Pr>::value && cp_eval<Dict, Pr>::ok &&
+                  pt_counted<Dict, Pr>::type::size == 1) {
+        // one adder form of the count, then the predicate as one function of its four words
+        // (WireWorld's "1 or 2 heads": three ops per 32 bits instead of five)
+        constexpr long long K = kfirst<typename pt_counted<Dict, Pr>::type>::value;
+        moore_sum ms;
+        if constexpr (SelfFree) ms = moore_sum_of<K>(self_incl_window<Win>{w, w.t});
+        else ms = moore_sum_of<K>(w);
+        const word in[4] = {ms.b0, ms.s1, ms.k, ms.m1};
+        return pt_fn<4, cp_table<Dict, Pr>(), 0ull>(in);
+    } else if constexpr (SelfFree) {
+        return ev<Dict, Pr>::at(self_incl_window<Win>{w, w.t});
+    } else {
+        return ev<Dict, Pr>::at(w);
+    }
 }
 
 // A boolean function of the first N inputs in[0..N-1] (truth table F, free entries D) from
