@@ -126,13 +126,14 @@ ALL = ["game-of-life", "wire", "forest-fire", "excitable", "brian", "maze", "tra
 
 @pytest.mark.parametrize("rule", ALL)
 def test_no_reachable_kernel_uses_scratch(rule):
-    """Every kernel the planner can launch (both layouts, the three lane mappings, the simple
-    traverser and every depth, and the batch kernels) keeps its registers out of scratch
-    memory; depths whose kernel would spill run at a shallower depth."""
+    """Every kernel the planner can launch (both layouts, the three lane mappings, the tiled
+    planes processed as rows at 4096 and 16384 cells wide, the simple traverser and every
+    depth, and the batch kernels) keeps its registers out of scratch memory; depths whose
+    kernel would spill run at a shallower depth."""
     from cellato_amd import _lib
     from cellato_amd.rules import rule as get_rule
     rid = get_rule(rule).id
-    for layout, widths in ((0, (448, 4096, 16384)), (1, (448, 512, 16384))):
+    for layout, widths in ((0, (448, 4096, 16384)), (1, (448, 512, 4096, 16384))):
         for width in widths:
             for T in range(0, 9):
                 eff, priv, regs = _lib.kernel_info(rid, layout, width, T)
