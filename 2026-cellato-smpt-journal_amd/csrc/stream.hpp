@@ -193,11 +193,13 @@ __device__ __forceinline__ word shift1(word own, uint32_t left_hi, uint32_t righ
 }
 
 // Linear layout: a word is 64 cells of one row; cell x at bit x % 64. Targets = the counted
-// states whose row sums the ring slots carry.
-template <int P, class Targets = klist<>>
+// states whose row sums the ring slots carry; VnTargets = those of them kept as (3-cell OR,
+// mask) for a von Neumann "any" test.
+template <int P, class Targets = klist<>, class VnTargets = klist<>>
 struct linear_window {
     static constexpr int NS = Targets::size;
     static constexpr bool row_sums = NS > 0;
+    using vn_targets = VnTargets;
     const lrow<P, NS>& top;
     const lrow<P, NS>& mid;
     const lrow<P, NS>& bot;
@@ -502,7 +504,9 @@ constexpr int rows_prefetch(int T) {
 // instead of one per plane followed by the mask of the exchanged halves); end lanes that load
 // the outside column (HALO) take its mask instead. A state the rule reads only as "some
 // neighbour is K" (Eval::row_sum_or_only) keeps the 3-cell OR in the sum word and 0 in the
-// carry word: one op instead of two, and a carry word that is a constant, not a register.
+// carry word: one op instead of two, and a carry word that is a constant, not a register. A
+// state read only as "some von Neumann neighbour is K" (Eval::row_sum_vn_any) keeps the 3-cell
+// OR and the mask itself, the mask of the cells above and below for the rows around.
 template <class Eval, class Targets>
 struct row_sum_fill;
 template <class Eval, long long... K>
@@ -531,7 +535,10 @@ struct row_sum_fill<Eval, klist<K...>> {
             lh = xc.left0(hi32(m));
             rl = xc.right0(lo32(m));
         }
-        if constexpr (Eval::template row_sum_or_only<Kj>) {
+        if constexpr (Eval::template row_sum_vn_any<Kj>) {
+            r.rs_s[j] = lop3<0xFE>(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl));
+            r.rs_c[j] = m;
+        } else if constexpr (Eval::template row_sum_or_only<Kj>) {
             r.rs_s[j] = lop3<0xFE>(shift1<-1>(m, lh, rl), m, shift1<1>(m, lh, rl));
             r.rs_c[j] = 0;
         } else {
@@ -874,7 +881,8 @@ __device__ __forceinline__ void stream_task(const pass_args& a, int task) {
                     // it): one compare and one select per stage (forming the row, y - k wrapped,
                     // took five more ops per word and stage)
                     const uint32_t yk = ypar ^ kflip[k] ^ ((hodd & (ycur < kmod[k])) ? ~0u : 0u);
-                    const linear_window<P, targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw], join(yk, yk), t};
+                    const linear_window<P, targets, typename Eval::vn_sum_targets> wnd{lwin[k][tp], lwin[k][md], lwin[k][nw],
+                                                                                        join(yk, yk), t};
                     out = Eval::evaluate(wnd);
                 } else {
                     // without edge words, the halo column (HALO_LOAD) is kept in stage 0's slots only

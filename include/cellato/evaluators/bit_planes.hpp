@@ -275,6 +275,18 @@ constexpr bool has_row_sums() {
     if constexpr (requires { W::row_sums; }) return W::row_sums;
     else return false;
 }
+// the von Neumann "any" states whose row sums a window keeps as (3-cell OR, mask), and whether
+// its counts include the cell itself
+template <class W>
+struct win_vn { using type = klist<>; };
+template <class W>
+    requires requires { typename W::vn_targets; }
+struct win_vn<W> { using type = typename W::vn_targets; };
+template <class W>
+constexpr bool includes_cell() {
+    if constexpr (requires { W::includes_cell; }) return W::includes_cell;
+    else return false;
+}
 
 // the Moore count (0..8) of a counted state around a cell, in adder form
 struct moore_sum {
@@ -661,7 +673,17 @@ template <class Dict, class C, auto Z>
 struct ev<Dict, greater_than<count_neighbors<C, von_neumann_4_neighbors>, constant<Z>>> {
     template <class Win>
     CELLATO_HD static word at(const Win& w) {
-        if constexpr (Z == 0) {
+        if constexpr (Z == 0 && is_state_literal<C>::value && includes_cell<Win>() &&
+                      kindex<index_of<Dict, C::value>, typename win_vn<Win>::type>::value >= 0) {
+            // the own row's 3-cell OR (the cell included: never K where this is read) and the
+            // masks of the cells above and below, all kept per row
+            constexpr long long K = index_of<Dict, C::value>;
+            word st, ct, sm, cm, sb, cb;
+            w.template rsum<K, -1>(st, ct);
+            w.template rsum<K, 0>(sm, cm);
+            w.template rsum<K, 1>(sb, cb);
+            return lop3<0xFE>(sm, ct, cb);
+        } else if constexpr (Z == 0) {
             word m[4];
             vn_masks<Dict, C>::get(w, m);
             return m[0] | m[1] | m[2] | m[3];
@@ -1116,6 +1138,43 @@ template <class Dict, auto V, auto Z, long long K>
 struct any_only<Dict, greater_than<count_neighbors<state_constant<V>, moore_8_neighbors>, constant<Z>>, K>
     : std::bool_constant<Z == 0 || index_of<Dict, V> != K> {};
 
+// whether state K is counted only as "some von Neumann neighbour is K" (no other count of K)
+template <class Dict, class E, long long K>
+struct vn_any_only : std::true_type {};
+template <class Dict, template <class...> class N, class... A, long long K>
+struct vn_any_only<Dict, N<A...>, K> : std::bool_constant<(vn_any_only<Dict, A, K>::value && ...)> {};
+template <class Dict, auto V, class Nb, long long K>
+struct vn_any_only<Dict, count_neighbors<state_constant<V>, Nb>, K> : std::bool_constant<index_of<Dict, V> != K> {};
+template <class Dict, auto V, auto Z, long long K>
+struct vn_any_only<Dict, greater_than<count_neighbors<state_constant<V>, von_neumann_4_neighbors>, constant<Z>>, K>
+    : std::bool_constant<Z == 0 || index_of<Dict, V> != K> {};
+// the states a rule counts over the von Neumann neighbourhood
+template <class Dict, class E>
+struct vn_targets { using type = klist<>; };
+template <class Dict, template <class...> class N, class... A>
+struct vn_targets<Dict, N<A...>> { using type = typename kjoin_all<typename vn_targets<Dict, A>::type...>::type; };
+template <class Dict, auto V>
+struct vn_targets<Dict, count_neighbors<state_constant<V>, von_neumann_4_neighbors>> { using type = klist<index_of<Dict, V>>; };
+
+// A von Neumann "any" state whose row sums a streaming caller may keep as (3-cell OR, mask):
+// read only as "some neighbour is K", through the predicate table and counting the cell
+// (the own row's OR then includes the cell, which is never K where it is read).
+template <class Dict, class E, int P, long long K>
+struct vn_sum_ok {
+    static constexpr bool value = [] {
+        if constexpr (P > 1 && pt_rule<Dict, E, P>::ok)
+            return vn_any_only<Dict, E, K>::value && pt_rule<Dict, E, P>::template self_free_counts<K>();
+        else
+            return false;
+    }();
+};
+template <class Dict, class E, int P, class L>
+struct vn_sum_list;
+template <class Dict, class E, int P, long long... K>
+struct vn_sum_list<Dict, E, P, klist<K...>> {
+    using type = typename kjoin_all<std::conditional_t<vn_sum_ok<Dict, E, P, K>::value, klist<K>, klist<>>...>::type;
+};
+
 // A window whose own cell matches nothing: a Moore count read through it includes the cell
 // (the own row's 3-cell sum is not reduced by the cell). Exact for a predicate that the rule
 // reads only at cells that are not the counted state, and two ops per 32 bits cheaper.
@@ -1124,6 +1183,8 @@ struct self_incl_window {
     const Win& w;
     int t;
     static constexpr bool row_sums = has_row_sums<Win>();
+    static constexpr bool includes_cell = true;
+    using vn_targets = typename win_vn<Win>::type;
     CELLATO_HD word yodd() const { return w.yodd(); }
     template <long long K, int dy>
     CELLATO_HD void rsum(word& s, word& c) const { w.template rsum<K, dy>(s, c); }
@@ -1272,11 +1333,18 @@ struct evaluator {
     // grid layout the evaluator reads (0 linear, 1 tiled 8x8): the HIP traversers check that
     // it matches the grid's
     static constexpr int layout = 0;
-    // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of
-    using row_sum_targets = typename row_targets<state_dictionary_type, Expression>::type;
+    // von Neumann "any" states whose row sums a streaming caller may keep as (3-cell OR, mask)
+    using vn_sum_targets = typename vn_sum_list<state_dictionary_type, Expression, planes,
+                                                typename vn_targets<state_dictionary_type, Expression>::type>::type;
+    // states whose Moore-neighbourhood count a streaming caller keeps per-row sums of, then
+    // the von Neumann "any" states above
+    using row_sum_targets =
+        typename kjoin<typename row_targets<state_dictionary_type, Expression>::type, vn_sum_targets>::type;
     // a counted state read only as "some neighbour is K", through the predicate table and
     // counting the cell itself: its row sum may be kept as the 3-cell OR alone (the sum word
     // holding the OR, the carry word 0), one word per ring slot instead of two
+    template <long long K>
+    static constexpr bool row_sum_vn_any = kindex<K, vn_sum_targets>::value >= 0;
     template <long long K>
     static constexpr bool row_sum_or_only = [] {
         if constexpr (planes > 1 && pt_rule<state_dictionary_type, Expression, planes>::ok)

@@ -55,6 +55,13 @@ constexpr bool or_sums = bp::evaluator<std::uint64_t, typename cellato::automata
                                        typename cellato::automata::by_id<id>::config::algorithm>::template row_sum_or_only<K>;
 static_assert(or_sums<cellato::automata::EXCITABLE, 1> && !or_sums<cellato::automata::WIRE, 1> &&
               !or_sums<cellato::automata::BRIAN, 2> && !or_sums<cellato::automata::GAME_OF_LIFE, 1>);
+// forest fire reads its burning von Neumann neighbours only at trees and ash: its fire mask is
+// kept per row as (3-cell OR, mask); the windows of this test keep none, so the masks path runs
+template <int id>
+using vn_sums = typename bp::evaluator<std::uint64_t, typename cellato::automata::by_id<id>::config::state_dictionary,
+                                       typename cellato::automata::by_id<id>::config::algorithm>::vn_sum_targets;
+static_assert(std::is_same_v<vn_sums<cellato::automata::FOREST_FIRE>, bp::klist<2>> &&
+              std::is_same_v<vn_sums<cellato::automata::WIRE>, bp::klist<>>);
 
 using namespace cellato::evaluators::bit_planes;
 using u64 = std::uint64_t;
