@@ -746,6 +746,114 @@ struct ev<Dict, margolus_180_neighbor> {
     }
 };
 
+// ---- Margolus table path ----------------------------------------------------------------
+// A one-plane rule made of the cell, one state's Margolus block count, the 180-degree partner,
+// constants, comparisons, and / or / not and if-then-else (Critters) depends only on the four
+// cells of the cell's 2x2 block: the cell, its vertical partner, its horizontal partner and
+// the diagonal one (which is also the 180-degree partner). mg_val<..>::at(cell, count,
+// partner) evaluates such a rule at compile time; the kernel then forms the four block words
+// and each result plane is one 4-input function of them (three ops per 32 bits) instead of
+// the 4-way count, its comparisons and the selects.
+template <class Dict, class E>
+struct mg_val { static constexpr bool ok = false; static constexpr long long K = -1;
+                static constexpr long long at(long long, long long, long long) { return 0; } };
+template <class Dict, auto V>
+struct mg_val<Dict, state_constant<V>> {
+    static constexpr bool ok = true; static constexpr long long K = -1;
+    static constexpr long long at(long long, long long, long long) { return index_of<Dict, V>; }
+};
+template <class Dict, auto V>
+struct mg_val<Dict, constant<V>> {
+    static constexpr bool ok = true; static constexpr long long K = -1;
+    static constexpr long long at(long long, long long, long long) { return static_cast<long long>(V); }
+};
+template <class Dict>
+struct mg_val<Dict, neighbor_at<0, 0>> {
+    static constexpr bool ok = true; static constexpr long long K = -1;
+    static constexpr long long at(long long c, long long, long long) { return c; }
+};
+template <class Dict>
+struct mg_val<Dict, margolus_180_neighbor> {
+    static constexpr bool ok = true; static constexpr long long K = -1;
+    static constexpr long long at(long long, long long, long long r) { return r; }
+};
+template <class Dict, auto V>
+struct mg_val<Dict, count_neighbors<state_constant<V>, margolus_alternating_neighborhood>> {
+    static constexpr bool ok = true; static constexpr long long K = index_of<Dict, V>;
+    static constexpr long long at(long long, long long n, long long) { return n; }
+};
+// the counted state of two subtrees: -1 none, -2 two different ones
+constexpr long long mg_k(long long a, long long b) { return a == -1 ? b : b == -1 ? a : a == b ? a : -2; }
+#define CELLATO_MG_BIN(node, expr)                                                              \
+    template <class Dict, class L, class R>                                                     \
+    struct mg_val<Dict, node<L, R>> {                                                           \
+        static constexpr bool ok = mg_val<Dict, L>::ok && mg_val<Dict, R>::ok;                  \
+        static constexpr long long K = mg_k(mg_val<Dict, L>::K, mg_val<Dict, R>::K);            \
+        static constexpr long long at(long long c, long long n, long long r) {                  \
+            const long long a = mg_val<Dict, L>::at(c, n, r), b = mg_val<Dict, R>::at(c, n, r); \
+            return expr;                                                                        \
+        }                                                                                       \
+    };
+CELLATO_MG_BIN(equals, a == b ? 1 : 0)
+CELLATO_MG_BIN(not_equals, a != b ? 1 : 0)
+CELLATO_MG_BIN(greater_than, a > b ? 1 : 0)
+CELLATO_MG_BIN(less_than, a < b ? 1 : 0)
+CELLATO_MG_BIN(and_, (a && b) ? 1 : 0)
+CELLATO_MG_BIN(or_, (a || b) ? 1 : 0)
+#undef CELLATO_MG_BIN
+template <class Dict, class V>
+struct mg_val<Dict, not_<V>> {
+    static constexpr bool ok = mg_val<Dict, V>::ok; static constexpr long long K = mg_val<Dict, V>::K;
+    static constexpr long long at(long long c, long long n, long long r) { return mg_val<Dict, V>::at(c, n, r) ? 0 : 1; }
+};
+template <class Dict, class C, class A, class B>
+struct mg_val<Dict, if_then_else<C, A, B>> {
+    static constexpr bool ok = mg_val<Dict, C>::ok && mg_val<Dict, A>::ok && mg_val<Dict, B>::ok;
+    static constexpr long long K = mg_k(mg_k(mg_val<Dict, C>::K, mg_val<Dict, A>::K), mg_val<Dict, B>::K);
+    static constexpr long long at(long long c, long long n, long long r) {
+        return mg_val<Dict, C>::at(c, n, r) ? mg_val<Dict, A>::at(c, n, r) : mg_val<Dict, B>::at(c, n, r);
+    }
+};
+
+template <class Dict, class E>
+struct mg_rule {
+    using X = mg_val<Dict, E>;
+    static constexpr bool ok = X::ok && X::K >= 0 && Dict::needed_bits == 1;
+    // result bit over the block cells' bits, index own + 2 ver + 4 own_h + 8 ver_h
+    static constexpr unsigned long long table() {
+        unsigned long long t = 0;
+        for (int i = 0; i < 16; ++i) {
+            const long long own = i & 1, ver = (i >> 1) & 1, own_h = (i >> 2) & 1, ver_h = (i >> 3) & 1;
+            const long long n = (own == X::K) + (ver == X::K) + (own_h == X::K) + (ver_h == X::K);
+            if (X::at(own, n, ver_h) & 1) t |= 1ull << i;
+        }
+        return t;
+    }
+};
+
+struct plane0 {
+    template <int N> CELLATO_HD word operator()(const vint<N>& v) const { return v.p[0]; }
+};
+
+template <int N, unsigned long long F, unsigned long long D>
+CELLATO_HD word pt_fn(const word* in);
+
+template <class Dict, class E, class Win>
+CELLATO_HD word mg_eval(const Win& w) {
+    const plane0 f{};
+    word in[4];
+    in[0] = w.template nbf<0, 0>(f);
+    in[1] = vpartner_f<0>(w, f);
+    if ((w.t & 1) == 0) {
+        in[2] = swap_pairs(in[0]);
+        in[3] = swap_pairs(in[1]);
+    } else {
+        in[2] = lop3<0xCA>(even_cols, w.template nbf<-1, 0>(f), w.template nbf<1, 0>(f));
+        in[3] = lop3<0xCA>(even_cols, vpartner_f<-1>(w, f), vpartner_f<1>(w, f));
+    }
+    return pt_fn<4, mg_rule<Dict, E>::table(), 0ull>(in);
+}
+
 // ---- rule traits: which kinds of nodes a rule uses (compile-time walk of the AST) -----------
 enum rule_trait : unsigned {
     TRAIT_MOORE = 1u,          // count over the Moore neighbourhood
@@ -1360,7 +1468,11 @@ struct evaluator {
 
     template <class Win>
     CELLATO_HD static vint<planes> evaluate(const Win& w) {
-        if constexpr (planes == 1 && has_row_sums<Win>() && cc_rule<state_dictionary_type, Expression>::ok) {
+        if constexpr (planes == 1 && mg_rule<state_dictionary_type, Expression>::ok) {
+            vint<1> r;
+            r.p[0] = mg_eval<state_dictionary_type, Expression>(w);
+            return r;
+        } else if constexpr (planes == 1 && has_row_sums<Win>() && cc_rule<state_dictionary_type, Expression>::ok) {
             vint<1> r;
             r.p[0] = cc_eval<state_dictionary_type, Expression>(w);
             return r;

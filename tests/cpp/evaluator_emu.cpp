@@ -57,6 +57,15 @@ static_assert(or_sums<cellato::automata::EXCITABLE, 1> && !or_sums<cellato::auto
               !or_sums<cellato::automata::BRIAN, 2> && !or_sums<cellato::automata::GAME_OF_LIFE, 1>);
 // forest fire reads its burning von Neumann neighbours only at trees and ash: its fire mask is
 // kept per row as (3-cell OR, mask); the windows of this test keep none, so the masks path runs
+// Critters takes the Margolus table path; its table over (cell, vertical, horizontal, diagonal
+// partner) bits, written out from the rule by hand: count 3 -> dead if the diagonal (180-degree)
+// partner is alive else alive, count 2 -> the cell, otherwise the cell toggled
+template <int id>
+using mg_of = bp::mg_rule<typename cellato::automata::by_id<id>::config::state_dictionary,
+                          typename cellato::automata::by_id<id>::config::algorithm>;
+static_assert(mg_of<cellato::automata::CRITTERS>::ok && mg_of<cellato::automata::CRITTERS>::table() == 0x3BD);
+static_assert(!mg_of<cellato::automata::GAME_OF_LIFE>::ok && !mg_of<cellato::automata::MAZE>::ok &&
+              !mg_of<cellato::automata::TRAFFIC>::ok);
 template <int id>
 using vn_sums = typename bp::evaluator<std::uint64_t, typename cellato::automata::by_id<id>::config::state_dictionary,
                                        typename cellato::automata::by_id<id>::config::algorithm>::vn_sum_targets;
