@@ -226,8 +226,10 @@ CELLATO_HD vint<3> count4(const word (&m)[4]) {
 // x-partner inside 2-cell pairs (cells 2j <-> 2j+1 of a word); valid for both layouts since
 // every word starts at an even x.
 CELLATO_HD word swap_pairs(word v) {
-    constexpr word even = 0x5555555555555555ULL;
-    return ((v & even) << 1) | ((v >> 1) & even);
+    // per 32-bit half (pairs never straddle the halves): even cells from the shifted-down
+    // word, odd cells from the shifted-up one, one select
+    const auto half = [](std::uint32_t x) -> std::uint32_t { return ((x >> 1) & 0x55555555u) | ((x << 1) & 0xAAAAAAAAu); };
+    return (word(half(static_cast<std::uint32_t>(v >> 32))) << 32) | half(static_cast<std::uint32_t>(v));
 }
 
 // ---- per-row sums of the counted states (streaming callers) --------------------------------
